@@ -1,0 +1,132 @@
+"""ctypes binding of libgatk_pairhmm.so (the C-ABI declared in include/gatk_pairhmm.h).
+
+The library is built in-tree by ``gatk_amd/csrc/Makefile`` (``__graft_entry__.build()``).  There is
+no CPU fallback: if the library is missing, loading raises ``NativeLibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgatk_pairhmm.so")
+
+PAIRHMM_OK = 0
+PAIRHMM_ERR_NO_DEVICE = 1
+PAIRHMM_ERR_BAD_ARG = 2
+PAIRHMM_ERR_OOM = 3
+PAIRHMM_ERR_KERNEL = 4
+PAIRHMM_ERR_STATE = 5
+
+RUN_PERSISTENT = 0
+RUN_PERSISTENT_EAGER = 1
+RUN_FP64 = 2
+
+# Every symbol include/gatk_pairhmm.h declares.
+EXPORTED_SYMBOLS = (
+    "pairhmm_load",
+    "pairhmm_initialize",
+    "pairhmm_compute_likelihoods",
+    "pairhmm_done",
+    "pairhmm_last_error",
+    "pairhmm_batch_upload",
+    "pairhmm_batch_forward",
+    "pairhmm_batch_free",
+)
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class ReadDataHolder(ctypes.Structure):
+    _fields_ = [
+        ("read_bases", ctypes.c_void_p),
+        ("read_quals", ctypes.c_void_p),
+        ("insertion_gop", ctypes.c_void_p),
+        ("deletion_gop", ctypes.c_void_p),
+        ("overall_gcp", ctypes.c_void_p),
+        ("length", ctypes.c_int32),
+    ]
+
+
+class HaplotypeDataHolder(ctypes.Structure):
+    _fields_ = [("haplotype_bases", ctypes.c_void_p), ("length", ctypes.c_int32)]
+
+
+class NativeArguments(ctypes.Structure):
+    _fields_ = [("max_number_of_threads", ctypes.c_int32), ("use_double_precision", ctypes.c_int32)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_reads", ctypes.c_int64),
+        ("read_bases", ctypes.c_void_p),
+        ("read_quals", ctypes.c_void_p),
+        ("read_ins", ctypes.c_void_p),
+        ("read_del", ctypes.c_void_p),
+        ("read_gcp", ctypes.c_void_p),
+        ("read_off", ctypes.c_void_p),
+        ("read_len", ctypes.c_void_p),
+        ("read_bytes", ctypes.c_int64),
+        ("n_haps", ctypes.c_int64),
+        ("hap_bases", ctypes.c_void_p),
+        ("hap_off", ctypes.c_void_p),
+        ("hap_len", ctypes.c_void_p),
+        ("hap_bytes", ctypes.c_int64),
+        ("n_pairs", ctypes.c_int64),
+        ("pair_read", ctypes.c_void_p),
+        ("pair_hap", ctypes.c_void_p),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("total_ms", ctypes.c_double),
+        ("fp32_kernel_ms", ctypes.c_double),
+        ("rescue_kernel_ms", ctypes.c_double),
+        ("n_rescued", ctypes.c_int64),
+        ("cells", ctypes.c_int64),
+    ]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load the in-tree native library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32 = ctypes.c_void_p, ctypes.c_int32
+        L.pairhmm_load.argtypes = []
+        L.pairhmm_load.restype = i32
+        L.pairhmm_initialize.argtypes = [ctypes.POINTER(NativeArguments), ctypes.POINTER(vp)]
+        L.pairhmm_initialize.restype = i32
+        L.pairhmm_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
+                                                  ctypes.POINTER(HaplotypeDataHolder), i32, vp]
+        L.pairhmm_compute_likelihoods.restype = i32
+        L.pairhmm_done.argtypes = [vp]
+        L.pairhmm_done.restype = i32
+        L.pairhmm_last_error.argtypes = []
+        L.pairhmm_last_error.restype = ctypes.c_char_p
+        L.pairhmm_batch_upload.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]
+        L.pairhmm_batch_upload.restype = i32
+        L.pairhmm_batch_forward.argtypes = [vp, vp, i32, vp, ctypes.POINTER(RunStats)]
+        L.pairhmm_batch_forward.restype = i32
+        L.pairhmm_batch_free.argtypes = [vp]
+        L.pairhmm_batch_free.restype = i32
+        _lib = L
+        return L
+
+
+def last_error() -> str:
+    msg = load_library().pairhmm_last_error()
+    return msg.decode("utf-8", "replace") if msg else ""

@@ -1,0 +1,542 @@
+// pairhmm_capi.cpp -- C-ABI of the gfx950 PairHMM engine: contexts, validation, staging, batches.
+//
+// Drop-in boundary: PairHMMNativeBinding as called by
+// src/main/java/org/broadinstitute/hellbender/utils/pairhmm/VectorLoglessPairHMM.java:57-168.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gatk_pairhmm.h"
+#include "model_tables.hpp"
+#include "pairhmm_kernels.hpp"
+
+using namespace gatk_pairhmm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+
+#define HIP_TRY(expr, status)                                                                           \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail((status), std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Underflow threshold on the FP32 scaled last-row sum (scale 2^120): a pair whose sum is below it
+// is recomputed in FP64.  Below 1e-28 the dominant paths' partial sums approach the FP32 normal
+// range (1.2e-38) and lose relative precision.
+constexpr double kFp32RescueThreshold = 1e-28;
+
+bool device_is_gfx950(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 4096);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct HostPinned {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 4096);
+        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct pairhmm_context {
+    int device = 0;
+    int num_cus = 0;
+    bool use_double = false;
+    hipStream_t stream = nullptr;
+    double *d_qual_err = nullptr;
+    double *d_mm_prob = nullptr;
+    DevBuf stage_dev;    // packed inputs of one computeLikelihoods call + outputs + counters
+    HostPinned stage_host;
+    int waves_per_block = 4;
+};
+
+struct pairhmm_batch {
+    pairhmm_context *ctx = nullptr;
+    int64_t n_pairs = 0;
+    int64_t cells = 0;
+    int max_hap_len = 1;
+    void *d_mem = nullptr;  // one allocation holding every device array of the batch
+    DeviceBatch view{};
+    uint32_t *d_order = nullptr;    // work queue: pair indices by descending cost
+    uint32_t *d_rescue = nullptr;   // rescue list (n_pairs capacity)
+    uint32_t *d_counters = nullptr; // [0] queue head, [1] rescue count
+    hipGraphExec_t graph = nullptr;
+    LaunchShape sh32, sh64;
+    int fp32_grid = 0, fp64_grid = 0;
+};
+
+extern "C" {
+
+const char *pairhmm_last_error(void) { return g_last_error.c_str(); }
+
+int32_t pairhmm_load(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    return device_is_gfx950(dev) ? 1 : 0;
+}
+
+int32_t pairhmm_initialize(const pairhmm_native_arguments *args, pairhmm_context **out_ctx) {
+    if (out_ctx == nullptr) return fail(PAIRHMM_ERR_BAD_ARG, "out_ctx is null");
+    *out_ctx = nullptr;
+    if (!pairhmm_load()) return fail(PAIRHMM_ERR_NO_DEVICE, "no gfx950 device visible to this process");
+    pairhmm_context *ctx = new (std::nothrow) pairhmm_context();
+    if (!ctx) return fail(PAIRHMM_ERR_OOM, "context allocation failed");
+    ctx->use_double = args ? args->use_double_precision != 0 : false;
+    hipError_t e = hipGetDevice(&ctx->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    static const ModelTables tables;  // built once per process (C++ magic static)
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_qual_err, sizeof(tables.qual_err));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_mm_prob, sizeof(double) * tables.mm_prob.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->d_qual_err, tables.qual_err, sizeof(tables.qual_err), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->d_mm_prob, tables.mm_prob.data(), sizeof(double) * tables.mm_prob.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        pairhmm_done(ctx);
+        return fail(e == hipErrorOutOfMemory ? PAIRHMM_ERR_OOM : PAIRHMM_ERR_KERNEL,
+                    std::string("initialize: ") + hipGetErrorString(e));
+    }
+    *out_ctx = ctx;
+    return PAIRHMM_OK;
+}
+
+int32_t pairhmm_done(pairhmm_context *ctx) {
+    if (!ctx) return PAIRHMM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->stage_dev.release();
+    ctx->stage_host.release();
+    if (ctx->d_qual_err) (void)hipFree(ctx->d_qual_err);
+    if (ctx->d_mm_prob) (void)hipFree(ctx->d_mm_prob);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PAIRHMM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// PairHMM.computeReadLikelihoodGivenHaplotypeLog10 argument checks (PairHMM.java:284-292) and
+// PairHMMModel.qualToTransProbs sign checks (PairHMMModel.java:108-111).
+int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *ins, const uint8_t *del, const uint8_t *gcp,
+                      int64_t len, int64_t idx) {
+    if (len < 0) return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has negative length");
+    if (len == 0) return PAIRHMM_OK;
+    if (!b || !q || !ins || !del || !gcp)
+        return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has a null array");
+    for (int64_t i = 0; i < len; ++i) {
+        if (q[i] > kMaxQual)
+            return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + ": base quality 255 is out of range");
+        if ((ins[i] | del[i] | gcp[i]) & 0x80)
+            return fail(PAIRHMM_ERR_BAD_ARG,
+                        "read " + std::to_string(idx) + ": insertion/deletion/GCP quality cannot be negative");
+    }
+    return PAIRHMM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
+                                               int32_t n_reads, const pairhmm_haplotype_data_holder *haps,
+                                               int32_t n_haps, double *likelihoods) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null (initialize was not called)");
+    if (n_reads < 0 || n_haps < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative read or haplotype count");
+    if (n_reads == 0 || n_haps == 0) return PAIRHMM_OK;
+    if (!reads || !haps || !likelihoods) return fail(PAIRHMM_ERR_BAD_ARG, "null array argument");
+    int64_t read_bytes = 0, hap_bytes = 0;
+    int max_h = 1;
+    for (int32_t r = 0; r < n_reads; ++r) {
+        const pairhmm_read_data_holder &rd = reads[r];
+        int32_t st = validate_read(rd.read_bases, rd.read_quals, rd.insertion_gop, rd.deletion_gop, rd.overall_gcp,
+                                   rd.length, r);
+        if (st != PAIRHMM_OK) return st;
+        read_bytes += rd.length;
+    }
+    for (int32_t h = 0; h < n_haps; ++h) {
+        if (haps[h].length <= 0 || haps[h].haplotype_bases == nullptr)
+            return fail(PAIRHMM_ERR_BAD_ARG, "haplotype " + std::to_string(h) + " is empty");
+        hap_bytes += haps[h].length;
+        max_h = std::max(max_h, haps[h].length);
+    }
+    const int64_t n_pairs = static_cast<int64_t>(n_reads) * n_haps;
+    if (n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "too many pairs in one call");
+
+    // Staging layout (host pinned and device identical): 5 read planes, hap plane, offsets, lengths.
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t o_rb = take(read_bytes), o_rq = take(read_bytes), o_ri = take(read_bytes), o_rd = take(read_bytes),
+                 o_rg = take(read_bytes), o_hb = take(hap_bytes), o_roff = take(sizeof(int64_t) * n_reads),
+                 o_rlen = take(sizeof(int32_t) * n_reads), o_hoff = take(sizeof(int64_t) * n_haps),
+                 o_hlen = take(sizeof(int32_t) * n_haps);
+    const size_t in_bytes = off;
+    const size_t o_out = take(sizeof(double) * n_pairs), o_rescue = take(sizeof(uint32_t) * n_pairs),
+                 o_cnt = take(sizeof(uint32_t) * 4);
+    const size_t total = off;
+
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    if (ctx->stage_host.reserve(in_bytes) != hipSuccess) return fail(PAIRHMM_ERR_OOM, "pinned staging allocation failed");
+    if (ctx->stage_dev.reserve(total) != hipSuccess) return fail(PAIRHMM_ERR_OOM, "device staging allocation failed");
+    uint8_t *hs = static_cast<uint8_t *>(ctx->stage_host.ptr);
+    int64_t *roff = reinterpret_cast<int64_t *>(hs + o_roff);
+    int32_t *rlen = reinterpret_cast<int32_t *>(hs + o_rlen);
+    int64_t *hoff = reinterpret_cast<int64_t *>(hs + o_hoff);
+    int32_t *hlen = reinterpret_cast<int32_t *>(hs + o_hlen);
+    int64_t pos = 0;
+    for (int32_t r = 0; r < n_reads; ++r) {
+        const pairhmm_read_data_holder &rd = reads[r];
+        roff[r] = pos;
+        rlen[r] = rd.length;
+        if (rd.length > 0) {
+            std::memcpy(hs + o_rb + pos, rd.read_bases, rd.length);
+            std::memcpy(hs + o_rq + pos, rd.read_quals, rd.length);
+            std::memcpy(hs + o_ri + pos, rd.insertion_gop, rd.length);
+            std::memcpy(hs + o_rd + pos, rd.deletion_gop, rd.length);
+            std::memcpy(hs + o_rg + pos, rd.overall_gcp, rd.length);
+        }
+        pos += rd.length;
+    }
+    pos = 0;
+    for (int32_t h = 0; h < n_haps; ++h) {
+        hoff[h] = pos;
+        hlen[h] = haps[h].length;
+        std::memcpy(hs + o_hb + pos, haps[h].haplotype_bases, haps[h].length);
+        pos += haps[h].length;
+    }
+    uint8_t *ds = static_cast<uint8_t *>(ctx->stage_dev.ptr);
+    HIP_TRY(hipMemcpyAsync(ds, hs, in_bytes, hipMemcpyHostToDevice, ctx->stream), PAIRHMM_ERR_KERNEL);
+
+    DeviceBatch b{};
+    b.hap_bases = ds + o_hb;
+    b.hap_off = reinterpret_cast<const int64_t *>(ds + o_hoff);
+    b.hap_len = reinterpret_cast<const int32_t *>(ds + o_hlen);
+    b.rd_bases = ds + o_rb;
+    b.rd_quals = ds + o_rq;
+    b.rd_ins = ds + o_ri;
+    b.rd_del = ds + o_rd;
+    b.rd_gcp = ds + o_rg;
+    b.rd_off = reinterpret_cast<const int64_t *>(ds + o_roff);
+    b.rd_len = reinterpret_cast<const int32_t *>(ds + o_rlen);
+    b.pair_read = nullptr;  // implicit read-major pairs
+    b.pair_hap = nullptr;
+    b.n_haps = n_haps;
+    b.out = reinterpret_cast<double *>(ds + o_out);
+    DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
+    uint32_t *counters = reinterpret_cast<uint32_t *>(ds + o_cnt);
+    uint32_t *rescue = reinterpret_cast<uint32_t *>(ds + o_rescue);
+
+    const bool fp64_all = ctx->use_double;
+    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block);
+    const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
+    if (sh64.waves_per_block == 0 || sh32.waves_per_block == 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
+    const int cus = std::max(ctx->num_cus, 1);
+    auto grid_for = [&](const LaunchShape &sh) {
+        const int64_t need = (n_pairs + sh.waves_per_block - 1) / sh.waves_per_block;
+        return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, static_cast<int64_t>(sh.blocks_per_cu) * cus)));
+    };
+    WorkSource all{nullptr, static_cast<uint32_t>(n_pairs), nullptr, nullptr};
+    RescueSink none{nullptr, nullptr, 0.0};
+    if (fp64_all) {
+        HIP_TRY(launch_forward(true, b, tab, all, none, max_h, sh64, grid_for(sh64), ctx->stream), PAIRHMM_ERR_KERNEL);
+    } else {
+        HIP_TRY(launch_zero_counters(counters, 4, ctx->stream), PAIRHMM_ERR_KERNEL);
+        RescueSink rs{rescue, counters + 1, kFp32RescueThreshold};
+        HIP_TRY(launch_forward(false, b, tab, all, rs, max_h, sh32, grid_for(sh32), ctx->stream), PAIRHMM_ERR_KERNEL);
+        // FP64 recompute of the flagged pairs; the count is read on the device.
+        WorkSource again{rescue, 0, counters + 1, nullptr};
+        HIP_TRY(launch_forward(true, b, tab, again, none, max_h, sh64, grid_for(sh64), ctx->stream), PAIRHMM_ERR_KERNEL);
+    }
+    HIP_TRY(hipMemcpyAsync(likelihoods, b.out, sizeof(double) * n_pairs, hipMemcpyDeviceToHost, ctx->stream),
+            PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
+    return PAIRHMM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batch API
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batch_desc *d, pairhmm_batch **out) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!d || !out) return fail(PAIRHMM_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    if (d->n_reads < 0 || d->n_haps <= 0 || d->n_pairs < 0 || d->read_bytes < 0 || d->hap_bytes <= 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
+    if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "batch exceeds 2^32 pairs");
+    if (d->n_reads > 0 && (!d->read_off || !d->read_len)) return fail(PAIRHMM_ERR_BAD_ARG, "null read tables");
+    if (!d->hap_bases || !d->hap_off || !d->hap_len) return fail(PAIRHMM_ERR_BAD_ARG, "null haplotype tables");
+    if (d->n_pairs > 0 && (!d->pair_read || !d->pair_hap)) return fail(PAIRHMM_ERR_BAD_ARG, "null pair tables");
+    int max_h = 1;
+    for (int64_t h = 0; h < d->n_haps; ++h) {
+        if (d->hap_len[h] <= 0 || d->hap_off[h] < 0 || d->hap_off[h] + d->hap_len[h] > d->hap_bytes)
+            return fail(PAIRHMM_ERR_BAD_ARG, "haplotype " + std::to_string(h) + " is empty or out of range");
+        max_h = std::max(max_h, d->hap_len[h]);
+    }
+    for (int64_t r = 0; r < d->n_reads; ++r) {
+        if (d->read_len[r] < 0 || d->read_off[r] < 0 || d->read_off[r] + d->read_len[r] > d->read_bytes)
+            return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(r) + " is out of range");
+    }
+    if (d->read_bytes > 0) {
+        if (!d->read_bases || !d->read_quals || !d->read_ins || !d->read_del || !d->read_gcp)
+            return fail(PAIRHMM_ERR_BAD_ARG, "null read plane");
+        int32_t st = validate_read(d->read_bases, d->read_quals, d->read_ins, d->read_del, d->read_gcp, d->read_bytes, -1);
+        if (st != PAIRHMM_OK) return st;
+    }
+    // Work queue: pairs sorted by descending cost (stripes x (H + 64) wave-steps) with a counting
+    // sort, so long pairs start first and the tail of the persistent launch is short.
+    const int64_t np = d->n_pairs;
+    std::vector<uint32_t> order(static_cast<size_t>(np));
+    int64_t cells = 0;
+    {
+        std::vector<uint32_t> cost(static_cast<size_t>(np));
+        uint32_t max_cost = 0;
+        for (int64_t p = 0; p < np; ++p) {
+            const int32_t r = d->pair_read[p], h = d->pair_hap[p];
+            if (r < 0 || r >= d->n_reads || h < 0 || h >= d->n_haps)
+                return fail(PAIRHMM_ERR_BAD_ARG, "pair " + std::to_string(p) + " indexes outside the batch");
+            const int R = d->read_len[r], H = d->hap_len[h];
+            cells += static_cast<int64_t>(R) * H;
+            const uint32_t c = static_cast<uint32_t>((R + kWave - 1) / kWave) * static_cast<uint32_t>(H + kWave);
+            cost[p] = c;
+            max_cost = std::max(max_cost, c);
+        }
+        std::vector<int64_t> start(static_cast<size_t>(max_cost) + 2, 0);
+        for (int64_t p = 0; p < np; ++p) start[max_cost - cost[p] + 1]++;
+        for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+        for (int64_t p = 0; p < np; ++p) order[start[max_cost - cost[p]]++] = static_cast<uint32_t>(p);
+    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t rb = static_cast<size_t>(d->read_bytes), hb = static_cast<size_t>(d->hap_bytes);
+    const size_t o_rb = take(rb), o_rq = take(rb), o_ri = take(rb), o_rd = take(rb), o_rg = take(rb), o_hb = take(hb),
+                 o_roff = take(8 * d->n_reads), o_rlen = take(4 * d->n_reads), o_hoff = take(8 * d->n_haps),
+                 o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np),
+                 o_out = take(8 * np), o_res = take(4 * np), o_cnt = take(4 * 16);
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    pairhmm_batch *bt = new (std::nothrow) pairhmm_batch();
+    if (!bt) return fail(PAIRHMM_ERR_OOM, "batch allocation failed");
+    bt->ctx = ctx;
+    bt->n_pairs = np;
+    bt->cells = cells;
+    bt->max_hap_len = max_h;
+    if (hipMalloc(&bt->d_mem, off) != hipSuccess) {
+        delete bt;
+        return fail(PAIRHMM_ERR_OOM, "device allocation of " + std::to_string(off) + " bytes failed");
+    }
+    uint8_t *m = static_cast<uint8_t *>(bt->d_mem);
+    struct Cp {
+        size_t o;
+        const void *src;
+        size_t n;
+    } copies[] = {{o_rb, d->read_bases, rb}, {o_rq, d->read_quals, rb}, {o_ri, d->read_ins, rb},
+                  {o_rd, d->read_del, rb},   {o_rg, d->read_gcp, rb},   {o_hb, d->hap_bases, hb},
+                  {o_roff, d->read_off, 8 * (size_t)d->n_reads}, {o_rlen, d->read_len, 4 * (size_t)d->n_reads},
+                  {o_hoff, d->hap_off, 8 * (size_t)d->n_haps},   {o_hlen, d->hap_len, 4 * (size_t)d->n_haps},
+                  {o_pr, d->pair_read, 4 * (size_t)np},          {o_ph, d->pair_hap, 4 * (size_t)np},
+                  {o_ord, order.data(), 4 * (size_t)np}};
+    for (const Cp &c : copies) {
+        if (c.n == 0) continue;
+        hipError_t e = hipMemcpy(m + c.o, c.src, c.n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            pairhmm_batch_free(bt);
+            return fail(PAIRHMM_ERR_KERNEL, std::string("batch upload: ") + hipGetErrorString(e));
+        }
+    }
+    DeviceBatch &v = bt->view;
+    v.rd_bases = m + o_rb;
+    v.rd_quals = m + o_rq;
+    v.rd_ins = m + o_ri;
+    v.rd_del = m + o_rd;
+    v.rd_gcp = m + o_rg;
+    v.hap_bases = m + o_hb;
+    v.rd_off = reinterpret_cast<const int64_t *>(m + o_roff);
+    v.rd_len = reinterpret_cast<const int32_t *>(m + o_rlen);
+    v.hap_off = reinterpret_cast<const int64_t *>(m + o_hoff);
+    v.hap_len = reinterpret_cast<const int32_t *>(m + o_hlen);
+    v.pair_read = reinterpret_cast<const int32_t *>(m + o_pr);
+    v.pair_hap = reinterpret_cast<const int32_t *>(m + o_ph);
+    v.n_haps = static_cast<int32_t>(d->n_haps);
+    v.out = reinterpret_cast<double *>(m + o_out);
+    bt->d_order = reinterpret_cast<uint32_t *>(m + o_ord);
+    bt->d_rescue = reinterpret_cast<uint32_t *>(m + o_res);
+    bt->d_counters = reinterpret_cast<uint32_t *>(m + o_cnt);
+    bt->sh32 = forward_shape(false, max_h, ctx->waves_per_block);
+    bt->sh64 = forward_shape(true, max_h, ctx->waves_per_block);
+    if (bt->sh32.waves_per_block == 0 || bt->sh64.waves_per_block == 0) {
+        pairhmm_batch_free(bt);
+        return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
+    }
+    bt->fp32_grid = bt->sh32.blocks_per_cu * std::max(ctx->num_cus, 1);
+    bt->fp64_grid = bt->sh64.blocks_per_cu * std::max(ctx->num_cus, 1);
+    *out = bt;
+    return PAIRHMM_OK;
+}
+
+namespace {
+
+// Enqueues one full forward pass of the batch: counter reset, persistent FP32 kernel over the
+// queue, persistent FP64 kernel over the rescue list.
+hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
+    pairhmm_context *ctx = bt->ctx;
+    DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
+    hipError_t e = launch_zero_counters(bt->d_counters, 16, s);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[0], s);
+    WorkSource q{bt->d_order, static_cast<uint32_t>(bt->n_pairs), nullptr, bt->d_counters + 0};
+    RescueSink rs{bt->d_rescue, bt->d_counters + 1, kFp32RescueThreshold};
+    e = launch_forward(false, bt->view, tab, q, rs, bt->max_hap_len, bt->sh32, bt->fp32_grid, s);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], s);
+    WorkSource q2{bt->d_rescue, 0, bt->d_counters + 1, bt->d_counters + 2};
+    RescueSink none{nullptr, nullptr, 0.0};
+    e = launch_forward(true, bt->view, tab, q2, none, bt->max_hap_len, bt->sh64, bt->fp64_grid, s);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    return e;
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt, int32_t mode, double *out_host,
+                                         pairhmm_run_stats *stats) {
+    if (!ctx || !bt || bt->ctx != ctx) return fail(PAIRHMM_ERR_STATE, "batch does not belong to this context");
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    hipStream_t s = ctx->stream;
+    hipEvent_t ev[4] = {};
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    pairhmm_run_stats st{};
+    st.cells = bt->cells;
+    if (bt->n_pairs == 0) {
+        if (stats) *stats = st;
+        return PAIRHMM_OK;
+    }
+    if (mode == PAIRHMM_RUN_PERSISTENT) {
+        if (!bt->graph) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), PAIRHMM_ERR_KERNEL);
+            hipError_t e = enqueue_forward(bt, s, nullptr);
+            hipError_t e2 = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(PAIRHMM_ERR_KERNEL, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            e = hipGraphInstantiate(&bt->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(PAIRHMM_ERR_KERNEL, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipGraphLaunch(bt->graph, s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
+    } else if (mode == PAIRHMM_RUN_PERSISTENT_EAGER) {
+        HIP_TRY(enqueue_forward(bt, s, ev), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
+    } else if (mode == PAIRHMM_RUN_FP64) {
+        DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
+        HIP_TRY(launch_zero_counters(bt->d_counters, 16, s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
+        WorkSource q{bt->d_order, static_cast<uint32_t>(bt->n_pairs), nullptr, bt->d_counters + 0};
+        RescueSink none{nullptr, nullptr, 0.0};
+        HIP_TRY(launch_forward(true, bt->view, tab, q, none, bt->max_hap_len, bt->sh64, bt->fp64_grid, s),
+                PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
+    } else {
+        return fail(PAIRHMM_ERR_BAD_ARG, "unknown run mode");
+    }
+    uint32_t counters[4] = {};
+    HIP_TRY(hipMemcpyAsync(counters, bt->d_counters, sizeof(counters), hipMemcpyDeviceToHost, s), PAIRHMM_ERR_KERNEL);
+    if (out_host)
+        HIP_TRY(hipMemcpyAsync(out_host, bt->view.out, sizeof(double) * bt->n_pairs, hipMemcpyDeviceToHost, s),
+                PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(s), PAIRHMM_ERR_KERNEL);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[0], ev[3]) == hipSuccess) st.total_ms = ms;
+    if (mode == PAIRHMM_RUN_PERSISTENT_EAGER) {
+        if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.fp32_kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
+    }
+    st.n_rescued = (mode == PAIRHMM_RUN_FP64) ? bt->n_pairs : counters[1];
+    if (stats) *stats = st;
+    return PAIRHMM_OK;
+}
+
+extern "C" int32_t pairhmm_batch_free(pairhmm_batch *bt) {
+    if (!bt) return PAIRHMM_OK;
+    if (bt->ctx) (void)hipSetDevice(bt->ctx->device);
+    if (bt->ctx && bt->ctx->stream) (void)hipStreamSynchronize(bt->ctx->stream);
+    if (bt->graph) (void)hipGraphExecDestroy(bt->graph);
+    if (bt->d_mem) (void)hipFree(bt->d_mem);
+    delete bt;
+    return PAIRHMM_OK;
+}

@@ -1,0 +1,285 @@
+// pairhmm_fwd.hip -- gfx950 PairHMM forward kernels (FP32 bulk pass and FP64 pass).
+//
+// Model: LoglessPairHMM.subComputeReadLikelihoodGivenHaplotypeLog10
+// (src/main/java/org/broadinstitute/hellbender/utils/pairhmm/LoglessPairHMM.java:20-68):
+//   M(r,c) = prior(r,c) * (M(r-1,c-1) MM_r + I(r-1,c-1) GM_r + D(r-1,c-1) GM_r)
+//   I(r,c) = M(r-1,c) MI_r + I(r-1,c) II_r
+//   D(r,c) = M(r,c-1) MD_r + D(r,c-1) DD_r
+// with M(0,.) = I(0,.) = 0, D(0,.) = 2^S / H, column 0 zero, and
+// log10 L = log10(sum_c M(R,c) + I(R,c)) - S log10 2.  Row r uses the qualities of read base r-1
+// (PairHMMModel.java:169-171); priors follow LoglessPairHMM.java:84-92 with tristate correction.
+//
+// Mapping onto a CDNA wavefront (one read/haplotype pair per wave):
+//   * Rows are processed in stripes of 64: lane k owns read row 64 s + k + 1 of stripe s and keeps
+//     that row's six transition probabilities and two priors in registers.
+//   * At step t lane k computes column c = t - k (an anti-diagonal sweep).  Its upper neighbours
+//     (row above, columns c and c-1) are lane k-1's values from the last two steps; they arrive by
+//     DPP wave_shr:1, one v_mov_dpp per state.  Lane 0's upper neighbour is the last row of the
+//     previous stripe, read from an LDS boundary row; the DPP leaves lane 0's "old" operand (that
+//     LDS value) in place, so lane 0 needs no extra VALU work.
+//   * Lane 63 (or, on the last stripe, the lane holding row R) writes its cell back into the same
+//     LDS boundary row, 63 columns behind lane 0's read point, so one buffer serves all stripes.
+//   * Cells with c <= 0 or c > H are computed too (they are exact zeros, resp. never read by a real
+//     cell), which keeps the inner loop free of masks.
+//   * The last-row sum, the unscaling and log10 are done in FP64.  FP32 launches use S = 120 and
+//     flag pairs whose scaled sum is below the underflow threshold for an FP64 recompute (S = 1020).
+#include "pairhmm_kernels.hpp"
+
+namespace gatk_pairhmm {
+namespace {
+
+template <typename T>
+struct Scale;
+template <>
+struct Scale<float> {
+    static constexpr int kExp = 120;
+};
+template <>
+struct Scale<double> {
+    static constexpr int kExp = 1020;
+};
+
+constexpr int kDppWaveShr1 = 0x138;  // DPP_WF_SR1: lane i reads lane i-1; lane 0 has no source
+
+// Lane k receives `src` from lane k-1; lane 0 keeps `old`.
+__device__ __forceinline__ float wave_shr1(float old, float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, src), kDppWaveShr1, 0xF,
+                                                                 0xF, false));
+}
+__device__ __forceinline__ double wave_shr1(double old, double src) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
+    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), kDppWaveShr1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32), kDppWaveShr1,
+                                               0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__host__ __device__ constexpr int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+constexpr int kUnroll = 4;  // wave-steps per unrolled loop body
+
+template <typename T>
+struct WaveLds {
+    uint8_t *hap;  // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
+    T *bm, *bi, *bd;  // [cols]: boundary row, column c at kLdsPad + c
+};
+
+template <typename T>
+__device__ __forceinline__ WaveLds<T> carve(uint8_t *base, int cols) {
+    WaveLds<T> w;
+    w.hap = base;
+    T *p = reinterpret_cast<T *>(base + cols);
+    w.bm = p;
+    w.bi = p + cols;
+    w.bd = p + 2 * cols;
+    return w;
+}
+
+// One stripe-sequential pass over the pair.  Returns the scaled last-row sum (FP64, all lanes).
+template <typename T, bool kHapHasN>
+__device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
+                               const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
+                               const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
+                               const DeviceTables &tab) {
+    const int n_stripes = (R + kWave - 1) / kWave;
+    for (int s = 0; s < n_stripes; ++s) {
+        const int p = s * kWave + lane;
+        T MM = 0, GM = 0, MI = 0, II = 0, MD = 0, DD = 0, pm = 0, px = 0;
+        uint32_t base = 0;
+        if (p < R) {
+            base = rb[p];
+            const int q = rq[p], qi = ri[p], qd = rd[p], qc = rg[p];
+            const double eq = tab.qual_err[q], ei = tab.qual_err[qi], ed = tab.qual_err[qd], ec = tab.qual_err[qc];
+            const int mn = qi < qd ? qi : qd, mx = qi < qd ? qd : qi;
+            MM = static_cast<T>(tab.mm_prob[((mx * (mx + 1)) >> 1) + mn]);
+            MI = static_cast<T>(ei);
+            MD = static_cast<T>(ed);
+            GM = static_cast<T>(1.0 - ec);
+            II = static_cast<T>(ec);
+            DD = II;
+            pm = static_cast<T>(1.0 - eq);
+            px = static_cast<T>(eq / 3.0);
+            if (base == 'N') px = pm;
+        }
+        const bool last = (s == n_stripes - 1);
+        const int writer = last ? (R - 1 - s * kWave) : (kWave - 1);
+        T m1 = 0, i1 = 0, d1 = 0;  // own cell at step t-1 (column c-1)
+        T mu = 0, iu = 0, du = 0;  // row above at column c-1 (received at step t-1)
+        const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
+        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *wd = L.bd + kLdsPad - writer;
+        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *ud = L.bd + kLdsPad;
+        // Steps run to column H + 63 for lane 63, rounded up to the unroll width; the extra
+        // steps only touch columns > H.
+        const int steps = round_up(H + kWave, kUnroll);
+        for (int t0 = 0; t0 < steps; t0 += kUnroll) {
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int t = t0 + u;
+                const T nmu = wave_shr1(um[t], m1);
+                const T niu = wave_shr1(ui[t], i1);
+                const T ndu = wave_shr1(ud[t], d1);
+                const uint32_t hb = hp[t];
+                const bool match = (hb == base) || (kHapHasN && hb == 'N');
+                const T prior = match ? pm : px;
+                const T m = prior * (mu * MM + (iu + du) * GM);
+                const T i = nmu * MI + niu * II;
+                const T d = m1 * MD + d1 * DD;
+                mu = nmu;
+                iu = niu;
+                du = ndu;
+                m1 = m;
+                i1 = i;
+                d1 = d;
+                if (lane == writer) {
+                    wm[t] = m;
+                    wi[t] = i;
+                    wd[t] = d;
+                }
+            }
+        }
+    }
+    double acc = 0.0;
+    for (int c = 1 + lane; c <= H; c += kWave)
+        acc += static_cast<double>(L.bm[kLdsPad + c]) + static_cast<double>(L.bi[kLdsPad + c]);
+    return wave_sum(acc);
+}
+
+template <typename T>
+__device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, const RescueSink &rs, uint8_t *lds,
+                             int cols, int lane, uint32_t pair) {
+    int r, h;
+    if (b.pair_read) {
+        r = b.pair_read[pair];
+        h = b.pair_hap[pair];
+    } else {
+        r = static_cast<int>(pair / static_cast<uint32_t>(b.n_haps));
+        h = static_cast<int>(pair % static_cast<uint32_t>(b.n_haps));
+    }
+    const int H = b.hap_len[h];
+    const int R = b.rd_len[r];
+    const uint8_t *hap = b.hap_bases + b.hap_off[h];
+    const int64_t ro = b.rd_off[r];
+    WaveLds<T> L = carve<T>(lds, cols);
+
+    // Stage the haplotype (zero padded) and the row-0 boundary: M = I = 0, D = 2^S / H
+    // (LoglessPairHMM.java:30-36).
+    int has_n = 0;
+    for (int j = lane; j < cols; j += kWave) {
+        const int c = j - kLdsPad + 1;  // hap byte j belongs to column c
+        uint8_t v = 0;
+        if (c >= 1 && c <= H) {
+            v = hap[c - 1];
+            has_n |= (v == 'N');
+        }
+        L.hap[j] = v;
+    }
+    const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
+    for (int j = lane; j < cols; j += kWave) {
+        L.bm[j] = 0;
+        L.bi[j] = 0;
+        L.bd[j] = d0;
+    }
+    double sum;
+    if (__any(has_n))
+        sum = forward_pair<T, true>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+                                    b.rd_gcp + ro, tab);
+    else
+        sum = forward_pair<T, false>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+                                     b.rd_gcp + ro, tab);
+    if (lane == 0) {
+        const double scale_log10 = static_cast<double>(Scale<T>::kExp) * 0.30102999566398119521;
+        b.out[pair] = log10(sum) - scale_log10;
+        if (rs.list != nullptr && !(sum >= rs.threshold)) {
+            const uint32_t slot = atomicAdd(rs.count, 1u);
+            rs.list[slot] = pair;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTables tab, WorkSource ws, RescueSink rs,
+                                                      int cols) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int waves_per_block = blockDim.x / kWave;
+    const size_t per_wave = static_cast<size_t>(cols) + 3 * static_cast<size_t>(cols) * sizeof(T);
+    uint8_t *lds = lds_raw + per_wave * wave;
+    const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
+    if (ws.head) {
+        for (;;) {
+            uint32_t it = 0;
+            if (lane == 0) it = atomicAdd(ws.head, 1u);
+            it = __builtin_amdgcn_readfirstlane(it);
+            if (it >= n_items) break;
+            process_pair<T>(b, tab, rs, lds, cols, lane, ws.items ? ws.items[it] : it);
+        }
+    } else {
+        const uint32_t stride = gridDim.x * waves_per_block;
+        for (uint32_t it = blockIdx.x * waves_per_block + wave; it < n_items; it += stride)
+            process_pair<T>(b, tab, rs, lds, cols, lane, ws.items ? ws.items[it] : it);
+    }
+}
+
+__global__ void zero_counters_kernel(uint32_t *c, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
+}
+
+}  // namespace
+
+// Columns -64..H+63+kUnroll of the boundary row and haplotype, plus slack, rounded to 16.
+int lds_columns(int max_hap_len) { return round_up(max_hap_len + 2 * kLdsPad + 16, 16); }
+
+size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
+    const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
+    return cols + 3 * cols * (fp64 ? sizeof(double) : sizeof(float));
+}
+
+hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTables &tables, const WorkSource &work,
+                          const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,
+                          hipStream_t stream) {
+    if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
+    const int cols = lds_columns(max_hap_len);
+    const dim3 block(kWave * shape.waves_per_block);
+    if (fp64) {
+        hipLaunchKernelGGL(forward_kernel<double>, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables,
+                           work, rescue, cols);
+    } else {
+        hipLaunchKernelGGL(forward_kernel<float>, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables,
+                           work, rescue, cols);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+    return hipGetLastError();
+}
+
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
+    LaunchShape sh;
+    const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64);
+    int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
+    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) --wpb;
+    if (per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) return sh;
+    sh.waves_per_block = wpb;
+    sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
+    int blocks = 0;
+    hipError_t e;
+    if (fp64)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, forward_kernel<double>, kWave * wpb, sh.lds_bytes);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, forward_kernel<float>, kWave * wpb, sh.lds_bytes);
+    sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
+    return sh;
+}
+
+}  // namespace gatk_pairhmm

@@ -1,0 +1,73 @@
+// pairhmm_kernels.hpp -- device-side data views and launch helpers of the PairHMM forward kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gatk_pairhmm {
+
+constexpr int kWave = 64;      // CDNA wavefront width: one read row per lane
+constexpr int kLdsPad = 64;    // boundary-row / haplotype slots before column 0 (virtual columns)
+
+// Packed batch resident in device memory.  Reads are five byte planes sharing rd_off/rd_len;
+// haplotypes one plane.  When pair_read is null the pairs are implicit and read-major:
+// pair p = r * n_haps + h (the computeLikelihoods output layout).
+struct DeviceBatch {
+    const uint8_t *hap_bases;
+    const int64_t *hap_off;
+    const int32_t *hap_len;
+    const uint8_t *rd_bases, *rd_quals, *rd_ins, *rd_del, *rd_gcp;
+    const int64_t *rd_off;
+    const int32_t *rd_len;
+    const int32_t *pair_read;
+    const int32_t *pair_hap;
+    int32_t n_haps;  // used only for implicit pairs
+    double *out;     // log10 likelihood per pair
+};
+
+// Qual -> probability tables in device memory (see model_tables.hpp).
+struct DeviceTables {
+    const double *qual_err;  // 256 entries
+    const double *mm_prob;   // triangular matchToMatch table
+};
+
+// Where a launch takes its pairs from.
+struct WorkSource {
+    const uint32_t *items;        // item -> pair index; null: item is the pair index
+    uint32_t n_items;             // item count when n_items_dev is null
+    const uint32_t *n_items_dev;  // item count read on the device (e.g. the rescue count)
+    uint32_t *head;               // non-null: dynamic queue (atomic head); null: grid-stride
+};
+
+// FP32 launches append pairs whose scaled last-row sum is below `threshold` to `list`.
+struct RescueSink {
+    uint32_t *list;   // null: no rescue (FP64 launches)
+    uint32_t *count;
+    double threshold;
+};
+
+// Bytes of LDS one wavefront needs for haplotypes up to max_hap_len.
+int lds_columns(int max_hap_len);
+size_t lds_bytes_per_wave(int max_hap_len, bool fp64);
+
+constexpr size_t kMaxLdsPerBlock = 65536;  // dynamic LDS budget per workgroup
+
+// Launch geometry of the forward kernel for one batch.
+struct LaunchShape {
+    int waves_per_block = 0;  // 0: the haplotype is too long for one wave's LDS slice
+    int blocks_per_cu = 0;    // resident blocks per CU at this LDS size (occupancy API)
+    size_t lds_bytes = 0;     // dynamic LDS per block
+};
+
+// Largest waves-per-block <= requested whose LDS fits kMaxLdsPerBlock, and its occupancy.
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block);
+
+hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTables &tables, const WorkSource &work,
+                          const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,
+                          hipStream_t stream);
+
+// Zeroes n counters with a kernel (graph-capturable; no memset node).
+hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream);
+
+}  // namespace gatk_pairhmm

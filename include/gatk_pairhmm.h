@@ -1,0 +1,148 @@
+/*
+ * gatk_pairhmm.h -- C-ABI of the gfx950 PairHMM engine (libgatk_pairhmm.so).
+ *
+ * The first block is the drop-in boundary: it is exactly what GATK's native PairHMM binding calls
+ * (interface org.broadinstitute.gatk.nativebindings.pairhmm.PairHMMNativeBinding, external jar
+ * gatk-native-bindings 1.1.0, build.gradle:303), as evidenced by its only caller
+ * src/main/java/org/broadinstitute/hellbender/utils/pairhmm/VectorLoglessPairHMM.java:
+ *   load(File)                                     :63, :71   -> pairhmm_load
+ *   initialize(PairHMMNativeArguments)             :81        -> pairhmm_initialize
+ *   computeLikelihoods(ReadDataHolder[],
+ *                      HaplotypeDataHolder[], double[]) :138  -> pairhmm_compute_likelihoods
+ *   done()                                         :164       -> pairhmm_done
+ * A JNI shim maps those four Java methods onto these functions one to one.
+ *
+ * The later blocks are new entry points with no counterpart in the reference: a device-resident
+ * batch API (one persistent launch over many active regions) and the backward (adjoint) API.
+ *
+ * All functions return a pairhmm_status code; pairhmm_last_error() describes the last failure of
+ * the calling thread.  Pointers are borrowed for the duration of the call only.
+ */
+#ifndef GATK_PAIRHMM_H
+#define GATK_PAIRHMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The JNI shim turns PAIRHMM_ERR_NO_DEVICE into
+ * UserException.HardwareFeatureException (VectorLoglessPairHMM.java:64-66, 72-74) and anything
+ * else into GATKException. */
+enum pairhmm_status {
+    PAIRHMM_OK = 0,
+    PAIRHMM_ERR_NO_DEVICE = 1, /* no usable gfx950 device                                   */
+    PAIRHMM_ERR_BAD_ARG = 2,   /* invalid lengths / qualities (IllegalArgumentException)     */
+    PAIRHMM_ERR_OOM = 3,       /* device or host allocation failed                           */
+    PAIRHMM_ERR_KERNEL = 4,    /* a HIP launch or copy failed                                */
+    PAIRHMM_ERR_STATE = 5      /* call on a context in the wrong state (IllegalStateException) */
+};
+
+/* ReadDataHolder (VectorLoglessPairHMM.java:122-127): five byte arrays of equal length.
+ * Bases are ASCII bytes; qualities are raw Phred bytes (not +33).  As in PairHMMModel
+ * (PairHMMModel.java:108-111) insertion/deletion/GCP bytes >= 128 are negative Java bytes and
+ * are rejected; a base quality byte of 255 is rejected (QualityUtils.java:48 holds 255 entries). */
+typedef struct pairhmm_read_data_holder {
+    const uint8_t *read_bases;
+    const uint8_t *read_quals;
+    const uint8_t *insertion_gop;
+    const uint8_t *deletion_gop;
+    const uint8_t *overall_gcp;
+    int32_t length;
+} pairhmm_read_data_holder;
+
+/* HaplotypeDataHolder (VectorLoglessPairHMM.java:97-98). */
+typedef struct pairhmm_haplotype_data_holder {
+    const uint8_t *haplotype_bases;
+    int32_t length;
+} pairhmm_haplotype_data_holder;
+
+/* PairHMMNativeArguments (PairHMMNativeArgumentCollection.java:18-22). */
+typedef struct pairhmm_native_arguments {
+    int32_t max_number_of_threads; /* accepted and ignored: the device supplies the parallelism */
+    int32_t use_double_precision;  /* nonzero: every pair runs the FP64 kernel                  */
+} pairhmm_native_arguments;
+
+typedef struct pairhmm_context pairhmm_context;
+
+/* ---- drop-in boundary ------------------------------------------------------------------- */
+
+/* PairHMMNativeBinding.load(File): 1 if a gfx950 device is usable by this process, else 0. */
+int32_t pairhmm_load(void);
+
+/* PairHMMNativeBinding.initialize(PairHMMNativeArguments): creates a context (own HIP stream,
+ * device tables, staging buffers).  Contexts are independent and may be used from different
+ * threads concurrently; one context must not be used by two threads at once. */
+int32_t pairhmm_initialize(const pairhmm_native_arguments *args, pairhmm_context **out_ctx);
+
+/* PairHMMNativeBinding.computeLikelihoods(reads, haplotypes, likelihoods):
+ * likelihoods[r * n_haplotypes + h] = log10 P(read r | haplotype h) (read-major, the layout
+ * VectorLoglessPairHMM.java:131,141-152 reads).  Synchronous: returns after the results are in
+ * host memory.  FP32 cells with FP64 recompute of pairs whose FP32 sum falls under the underflow
+ * threshold, or FP64 throughout when use_double_precision was set.  Values are <= 0 or -inf. */
+int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
+                                    int32_t n_reads, const pairhmm_haplotype_data_holder *haplotypes,
+                                    int32_t n_haplotypes, double *likelihoods);
+
+/* PairHMMNativeBinding.done(): releases the context and everything it holds. */
+int32_t pairhmm_done(pairhmm_context *ctx);
+
+/* Message for the last non-OK status returned on the calling thread. */
+const char *pairhmm_last_error(void);
+
+/* ---- batch API (new; no counterpart in the reference) ------------------------------------ */
+
+/* A packed batch of many active regions.  Reads and haplotypes are concatenated byte planes
+ * indexed by per-item offsets and lengths; pair p scores read pair_read[p] against haplotype
+ * pair_hap[p] and its result lands in out[p].  All arrays are host memory. */
+typedef struct pairhmm_batch_desc {
+    int64_t n_reads;
+    const uint8_t *read_bases, *read_quals, *read_ins, *read_del, *read_gcp; /* read_bytes each */
+    const int64_t *read_off;
+    const int32_t *read_len;
+    int64_t read_bytes;
+    int64_t n_haps;
+    const uint8_t *hap_bases; /* hap_bytes */
+    const int64_t *hap_off;
+    const int32_t *hap_len;
+    int64_t hap_bytes;
+    int64_t n_pairs;
+    const int32_t *pair_read;
+    const int32_t *pair_hap;
+} pairhmm_batch_desc;
+
+typedef struct pairhmm_batch pairhmm_batch;
+
+/* Validates the batch, copies it into device memory (HBM-resident until pairhmm_batch_free) and
+ * builds the work queue (pairs ordered by descending cell cost). */
+int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batch_desc *desc, pairhmm_batch **out_batch);
+
+/* Run modes for pairhmm_batch_forward. */
+enum pairhmm_run_mode {
+    PAIRHMM_RUN_PERSISTENT = 0, /* persistent FP32 kernel fed by a device queue + FP64 rescue,
+                                   replayed from a captured hipGraph                          */
+    PAIRHMM_RUN_PERSISTENT_EAGER = 1, /* same kernels launched without the graph, each timed */
+    PAIRHMM_RUN_FP64 = 2              /* every pair through the FP64 kernel                   */
+};
+
+typedef struct pairhmm_run_stats {
+    double total_ms;        /* wall time of the device work (HIP events), excluding D2H        */
+    double fp32_kernel_ms;  /* duration of the FP32 forward kernel (eager mode only)           */
+    double rescue_kernel_ms;/* duration of the FP64 rescue kernel (eager mode only)            */
+    int64_t n_rescued;      /* pairs recomputed in FP64                                        */
+    int64_t cells;          /* sum over pairs of read length x haplotype length                */
+} pairhmm_run_stats;
+
+/* Runs the forward pass over the whole batch.  out_host (n_pairs doubles, may be NULL) receives
+ * log10 likelihoods in pair order; stats (may be NULL) receives timings. */
+int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *batch, int32_t mode, double *out_host,
+                              pairhmm_run_stats *stats);
+
+int32_t pairhmm_batch_free(pairhmm_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GATK_PAIRHMM_H */

@@ -1,0 +1,233 @@
+"""GPU parity of the gfx950 forward kernels against the FP64 oracle and the reference's golden
+vectors, through the C-ABI (drop-in path and batch path).
+
+Tolerances: FP32 cells with FP64 rescue must be within |d log10 L| <= 1e-4 of the FP64 oracle (the
+project bar) and within 1e-5 on the reference's 104 VectorPairHMMUnitTest rows (its own bar,
+VectorPairHMMUnitTest.java:100); the FP64 kernel must be within 1e-9 of the oracle.
+"""
+import math
+import threading
+
+import numpy as np
+import pytest
+
+from gatk_amd import _native, synth
+from gatk_amd.batch import BatchEngine, pack_regions
+from gatk_amd.pairhmm import (HaplotypeDataHolder, HipLoglessPairHMM, HipPairHmm, LikelihoodMatrix,
+                              PairHMMNativeArguments, Read, ReadDataHolder, StandardPairHMMInputScoreImputator)
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-4
+FP64_TOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def engine():
+    with BatchEngine() as eng:
+        yield eng
+
+
+@pytest.fixture(scope="module")
+def engine64():
+    with BatchEngine(PairHMMNativeArguments(1, True)) as eng:
+        yield eng
+
+
+def _binding(use_double=False):
+    b = HipPairHmm()
+    assert b.load(None)
+    b.initialize(PairHMMNativeArguments(1, use_double))
+    return b
+
+
+class _Imputator:
+    def __init__(self, ins, dels, gcp):
+        self.v = (ins, dels, gcp)
+
+    def impute(self, read):
+        from gatk_amd.pairhmm import PairHMMInputScoreImputation
+        return PairHMMInputScoreImputation(*self.v)
+
+
+@pytest.mark.parametrize("use_double", [False, True])
+def test_vector_pairhmm_testdata(testdata_rows, use_double):
+    """VectorPairHMMUnitTest.testLikelihoodsFromHaplotypesForAvailableImplementations
+    (VectorPairHMMUnitTest.java:30-106): one initialize + computeLog10Likelihoods per row, 1e-5."""
+    hmm = HipLoglessPairHMM(PairHMMNativeArguments(1, use_double))
+    try:
+        worst = 0.0
+        for row in testdata_rows:
+            hmm.initialize([row["hap"]], None, 0, 0)
+            read = Read(row["read"], row["qual"], row["ins"], row["dels"])
+            hmm.compute_log10_likelihoods(LikelihoodMatrix([row["hap"]]), [read],
+                                          _Imputator(row["ins"], row["dels"], row["gcp"]))
+            la = hmm.get_log_likelihood_array()
+            worst = max(worst, abs(la[0] - row["expected"]))
+        assert worst <= 1e-5
+    finally:
+        hmm.close()
+
+
+def _single_pair_batch(rows):
+    regions = [([r["hap"]], [(np.frombuffer(r["read"], np.uint8), r["qual"], r["ins"], r["dels"], r["gcp"])])
+               for r in rows]
+    return pack_regions(regions)
+
+
+def test_hmmresults_fp32_and_fp64(engine, engine64, hmmresults):
+    rows = hmmresults["Java"]
+    b = _single_pair_batch(rows)
+    want = oracle.logless_batch(b, n_threads=8)
+    out = np.empty(b.n_pairs)
+    engine.upload(b).forward(out=out)
+    assert np.max(np.abs(out - want)) <= FP32_TOL
+    assert np.max(np.abs(out - np.array([r["expected"] for r in rows]))) <= FP32_TOL
+    out64 = np.empty(b.n_pairs)
+    engine64.upload(b).forward(mode=_native.RUN_FP64, out=out64)
+    assert np.max(np.abs(out64 - want)) <= FP64_TOL
+    texts = ["%e" % v for v in out64]
+    assert sum(t == r["text"] for t, r in zip(texts, rows)) >= len(rows) - 1
+
+
+def test_plain_small_drop_in(hmmresults):
+    """The plain_small workload (64 reads x 16 haplotypes) through computeLikelihoods."""
+    b = synth.workload("plain_small")
+    reads, haps = b.region(0)
+    want = oracle.logless_batch(b, n_threads=8).reshape(len(reads), len(haps))
+    for use_double, tol in ((False, FP32_TOL), (True, FP64_TOL)):
+        binding = _binding(use_double)
+        try:
+            out = np.empty(len(reads) * len(haps))
+            binding.compute_likelihoods([ReadDataHolder(*r) for r in reads], [HaplotypeDataHolder(h) for h in haps], out)
+        finally:
+            binding.done()
+        assert np.max(np.abs(out.reshape(len(reads), len(haps)) - want)) <= tol
+
+
+def _edge_region(rng):
+    hap = rng.choice(list(b"ACGT"), 300).astype(np.uint8)
+    haps = [hap, hap[:1], hap[:20], hap[:64], hap[:65], np.frombuffer(b"ACGTNNACGTacgt" * 5, np.uint8)]
+    reads = []
+    for R in (0, 1, 2, 20, 63, 64, 65, 100, 127, 128, 129, 200, 250):
+        start = int(rng.integers(0, 300 - R + 1)) if R <= 300 else 0
+        bases = hap[start:start + R].copy()
+        if R > 5:
+            bases[R // 2] = ord("N")
+        if R > 10:
+            bases[3] = ord("a")
+        reads.append((bases, rng.integers(6, 41, R).astype(np.uint8), rng.integers(20, 60, R).astype(np.uint8),
+                      rng.integers(20, 60, R).astype(np.uint8), rng.integers(5, 20, R).astype(np.uint8)))
+    return haps, reads
+
+
+def test_edge_cases_match_oracle():
+    """Empty / 1-base reads, reads shorter and longer than the haplotype, stripe boundaries at
+    63/64/65/127/128/129 rows, 'N' wildcards in read and haplotype, lowercase bytes."""
+    rng = np.random.default_rng(5)
+    haps, reads = _edge_region(rng)
+    b = pack_regions([(haps, reads)])
+    want = oracle.logless_batch(b, n_threads=8)
+    for use_double, tol in ((False, FP32_TOL), (True, FP64_TOL)):
+        binding = _binding(use_double)
+        try:
+            out = np.empty(len(reads) * len(haps))
+            binding.compute_likelihoods([ReadDataHolder(*r) for r in reads], [HaplotypeDataHolder(h) for h in haps], out)
+        finally:
+            binding.done()
+        inf = np.isinf(want)
+        assert np.array_equal(np.isinf(out), inf)
+        assert np.all(out[inf] == -math.inf)  # the empty read
+        assert np.max(np.abs(out[~inf] - want[~inf])) <= tol
+        assert np.all(out <= 0)
+
+
+def test_bad_arguments_raise():
+    binding = _binding()
+    try:
+        ok = np.full(4, 30, np.uint8)
+        hap = [HaplotypeDataHolder(np.frombuffer(b"ACGTACGT", np.uint8))]
+        out = np.empty(1)
+        for bad in ("qual", "ins", "gcp"):
+            arrs = [np.frombuffer(b"ACGT", np.uint8), ok.copy(), ok.copy(), ok.copy(), ok.copy()]
+            if bad == "qual":
+                arrs[1][2] = 255
+            elif bad == "ins":
+                arrs[2][0] = 128
+            else:
+                arrs[4][3] = 200
+            with pytest.raises(ValueError):
+                binding.compute_likelihoods([ReadDataHolder(*arrs)], hap, out)
+        with pytest.raises(ValueError):
+            binding.compute_likelihoods([ReadDataHolder(np.frombuffer(b"ACGT", np.uint8), ok, ok, ok, ok)],
+                                        [HaplotypeDataHolder(np.zeros(0, np.uint8))], out)
+    finally:
+        binding.done()
+
+
+def test_batched_uneven_persistent_graph(engine):
+    b = synth.workload("batched_uneven", n_regions=24, seed=11)
+    dev = engine.upload(b)
+    outs = []
+    for _ in range(3):
+        o = np.empty(b.n_pairs)
+        dev.forward(mode=_native.RUN_PERSISTENT, out=o)
+        outs.append(o)
+    eager = np.empty(b.n_pairs)
+    st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=eager)
+    assert st.fp32_kernel_ms > 0
+    for o in outs[1:] + [eager]:
+        assert np.array_equal(o, outs[0])
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(b.n_pairs, size=min(4000, b.n_pairs), replace=False))
+    want = oracle.logless_batch(b, n_threads=8, pairs=sample)
+    assert np.max(np.abs(outs[0][sample] - want)) <= FP32_TOL
+    dev.free()
+
+
+def test_precision_stress_rescue(engine, engine64):
+    """STR-rich haplotypes, low base qualities, long reads: many pairs underflow FP32 and are
+    recomputed in FP64; the combined result stays within 1e-4 of the oracle."""
+    b = synth.workload("precision_stress", n_regions=2, seed=3)
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(b.n_pairs, size=min(300, b.n_pairs), replace=False))
+    want = oracle.logless_batch(b, n_threads=8, pairs=sample)
+    out = np.empty(b.n_pairs)
+    st = engine.upload(b).forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+    assert st.n_rescued > 0
+    finite = np.isfinite(want)
+    assert np.array_equal(np.isfinite(out[sample]), finite)
+    assert np.max(np.abs(out[sample][finite] - want[finite])) <= FP32_TOL
+    out64 = np.empty(b.n_pairs)
+    engine64.upload(b).forward(mode=_native.RUN_FP64, out=out64)
+    assert np.max(np.abs(out64[sample][finite] - want[finite])) <= FP64_TOL
+
+
+def test_concurrent_contexts():
+    """Independent contexts on two threads (Spark runs many engines per JVM,
+    HaplotypeCallerSpark.java:172-183) give the same answers as one context."""
+    b = synth.workload("plain_small")
+    reads, haps = b.region(0)
+    rh = [ReadDataHolder(*r) for r in reads]
+    hh = [HaplotypeDataHolder(h) for h in haps]
+    results, errors = {}, []
+
+    def work(k):
+        try:
+            binding = _binding()
+            out = np.empty(len(rh) * len(hh))
+            for _ in range(5):
+                binding.compute_likelihoods(rh, hh, out)
+            binding.done()
+            results[k] = out
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(3)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors
+    assert np.array_equal(results[0], results[1]) and np.array_equal(results[0], results[2])
