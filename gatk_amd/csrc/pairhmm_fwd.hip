@@ -194,11 +194,15 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
     else
         sum = forward_pair<T, false>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
                                      b.rd_gcp + ro, tab);
-    if (lane == 0) {
-        const double scale_log10 = static_cast<double>(Scale<T>::kExp) * 0.30102999566398119521;
-        b.out[pair] = log10(sum) - scale_log10;
-        if (rs.list != nullptr && !(sum >= rs.threshold)) {
-            const uint32_t slot = atomicAdd(rs.count, 1u);
+    // Every lane holds the same sum (butterfly reduction), so the result and the rescue decision are
+    // wave-uniform; all lanes store the same value to the same address rather than branching on
+    // the lane id (a lane-divergent branch here would sit next to the convergent work fetch below).
+    const double scale_log10 = static_cast<double>(Scale<T>::kExp) * 0.30102999566398119521;
+    b.out[pair] = log10(sum) - scale_log10;
+    if (rs.list != nullptr) {
+        const int need = __builtin_amdgcn_readfirstlane(static_cast<int>(!(sum >= rs.threshold)));
+        if (need) {
+            const uint32_t slot = __builtin_amdgcn_readfirstlane(atomicAdd(rs.count, lane == 0 ? 1u : 0u));
             rs.list[slot] = pair;
         }
     }
@@ -216,9 +220,9 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
     const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
     if (ws.head) {
         for (;;) {
-            uint32_t it = 0;
-            if (lane == 0) it = atomicAdd(ws.head, 1u);
-            it = __builtin_amdgcn_readfirstlane(it);
+            // All lanes issue the atomic (lane 0 adds 1, the rest add 0), so the fetch has no
+            // lane-divergent branch; lane 0's pre-increment value is the wave's item.
+            const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
             if (it >= n_items) break;
             process_pair<T>(b, tab, rs, lds, cols, lane, ws.items ? ws.items[it] : it);
         }
