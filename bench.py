@@ -64,7 +64,20 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-regions", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": x.numel() * 4, "bytes_written": y.numel() * 4,
+                          "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (regions shard across GPUs "
               f"with no exchange; multi-GPU runs are out of scope here)", file=sys.stderr)

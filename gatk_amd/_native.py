@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_batch_upload",
     "pairhmm_batch_forward",
     "pairhmm_batch_free",
+    "pairhmm_forward_backward",
 )
 
 
@@ -81,6 +82,37 @@ class BatchDesc(ctypes.Structure):
     ]
 
 
+class GradDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_reads", ctypes.c_int64),
+        ("n_haps", ctypes.c_int64),
+        ("n_pairs", ctypes.c_int64),
+        ("read_bytes", ctypes.c_int64),
+        ("read_bases", ctypes.c_void_p),
+        ("read_qb", ctypes.c_void_p),
+        ("read_qi", ctypes.c_void_p),
+        ("read_qd", ctypes.c_void_p),
+        ("read_qc", ctypes.c_void_p),
+        ("read_off", ctypes.c_void_p),
+        ("read_len", ctypes.c_void_p),
+        ("hap_bases", ctypes.c_void_p),
+        ("hap_off", ctypes.c_void_p),
+        ("hap_len", ctypes.c_void_p),
+        ("pair_read", ctypes.c_void_p),
+        ("pair_hap", ctypes.c_void_p),
+        ("pair_weight", ctypes.c_void_p),
+        ("max_read_len", ctypes.c_int32),
+        ("max_hap_len", ctypes.c_int32),
+        ("use_double", ctypes.c_int32),
+        ("out_log10_likelihood", ctypes.c_void_p),
+        ("grad_qb", ctypes.c_void_p),
+        ("grad_qi", ctypes.c_void_p),
+        ("grad_qd", ctypes.c_void_p),
+        ("grad_qc", ctypes.c_void_p),
+        ("grad_base_error", ctypes.c_void_p),
+    ]
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double),
@@ -123,6 +155,8 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_batch_forward.restype = i32
         L.pairhmm_batch_free.argtypes = [vp]
         L.pairhmm_batch_free.restype = i32
+        L.pairhmm_forward_backward.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
+        L.pairhmm_forward_backward.restype = i32
         _lib = L
         return L
 

@@ -1,0 +1,467 @@
+// pairhmm_bwd.hip -- gfx950 PairHMM forward + backward (adjoint) kernel.
+//
+// No counterpart in the reference: LoglessPairHMM (LoglessPairHMM.java:20-68) has no gradient.  This
+// kernel differentiates the same recurrence with respect to continuous per-position Phred
+// parameters: base quality qb, insertion GOP qi, deletion GOP qd and gap continuation qc, with
+//   e(q) = 10^(-q/10),  MM = 1 - e(qi) - e(qd),  MI = e(qi),  MD = e(qd),  GM = 1 - e(qc),
+//   II = DD = e(qc),  prior = 1 - e(qb) on a match, e(qb)/3 on a mismatch
+// (MM = 1 - e1 - e2 is the closed form PairHMMModelUnitTest.java:237-286 checks the integer table
+// against).  Outputs per pair: log10 L; per read position: w_p * d log10 L_p / d(qb, qi, qd, qc) and
+// d / d e(qb), accumulated over the pairs that share the read.
+//
+// Schedule (one pair per wave, the forward kernel's stripes and anti-diagonal skew):
+//   1. Forward over all stripes, top down, exactly as the forward kernel, saving for every stripe
+//      the relay row (the row above the stripe) and, every kTile wave-steps, every lane's state
+//      registers (a checkpoint), both in per-wave global scratch.
+//   2. Adjoint over the stripes bottom up.  Wave-steps run in reverse, so with the same skew lane k
+//      finds lane k+1's adjoints from one and two steps earlier (DPP wave_shl:1); lane 63 takes the
+//      row below the stripe from an LDS row written by lane 1 of the stripe underneath.  For each
+//      tile (right to left) the kernel restores its checkpoint, recomputes kTile forward steps into
+//      registers and then runs the kTile adjoint steps over them.
+//   Adjoints (X = prior * gM, S = I + D):
+//      gS(r,c) = GM_{r+1} X(r+1,c+1)
+//      gD(r,c) = gS + DD_r gD(r,c+1)
+//      gI(r,c) = gS + II_{r+1} gI(r+1,c)
+//      gM(r,c) = MM_{r+1} X(r+1,c+1) + MI_{r+1} gI(r+1,c) + MD_r gD(r,c+1)
+//   Row r+1's MM, GM, MI, II adjoints need row r's forward values, so lane k accumulates them for
+//   the lane below and they are moved down one lane when a stripe ends.  The last-row seed
+//   (d L / d M(R,c) = d L / d I(R,c) = 1) enters as a virtual row R+1 with gI = 1, X = 0 and
+//   MM = GM = 0, MI = II = 1, which also makes rows beyond R pass the seed up unchanged.
+//   Scaling: forward cells carry 2^120 (FP32) / 2^1020 (FP64); the FP32 adjoint seed is 2^sigma with
+//   sigma = 90 - ilogb(L_scaled) so every accumulated product stays inside FP32's range; pairs whose
+//   L_scaled is below 2^-36 go to the FP64 kernel.  Per-lane sums are FP32 (FP64 in the FP64 kernel);
+//   per-position results are scaled and accumulated in FP64 with atomics.
+#include "pairhmm_device.hpp"
+#include "pairhmm_kernels.hpp"
+
+namespace gatk_pairhmm {
+namespace {
+
+using namespace dev;
+
+template <typename T>
+struct AdjointScale;
+template <>
+struct AdjointScale<float> {
+    static constexpr int kTarget = 90;     // L_scaled * 2^sigma ~ 2^90
+    static constexpr int kMaxSigma = 126;  // seed must stay below FLT_MAX
+};
+template <>
+struct AdjointScale<double> {
+    static constexpr int kTarget = 0;
+    static constexpr int kMaxSigma = 0;
+};
+
+constexpr float kLog2_10_over_10 = 0.33219280948873623479f;
+
+__device__ __forceinline__ float phred_to_err(float q) { return exp2f(-kLog2_10_over_10 * q); }
+__device__ __forceinline__ double phred_to_err(double q) { return exp2(-0.33219280948873623479 * q); }
+
+template <typename T>
+struct Row {
+    T MM, GM, MI, II, MD, DD, pm, px;
+    uint32_t base;
+    bool n_read;
+};
+
+// Row parameters of read position p (row p + 1).  Active rows use the read; rows past the read
+// (p >= R) are the pass-through rows that carry the adjoint seed (MI = II = 1, everything else 0).
+template <typename T>
+__device__ __forceinline__ Row<T> load_row(const GradBatch &g, int64_t ro, int R, int p, bool is_row) {
+    Row<T> w{T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), 0u, false};
+    if (!is_row) return w;
+    if (p >= R) {
+        w.MI = T(1);
+        w.II = T(1);
+        return w;
+    }
+    const T ei = phred_to_err(static_cast<T>(g.rd_qi[ro + p]));
+    const T ed = phred_to_err(static_cast<T>(g.rd_qd[ro + p]));
+    const T ec = phred_to_err(static_cast<T>(g.rd_qc[ro + p]));
+    const T eb = phred_to_err(static_cast<T>(g.rd_qb[ro + p]));
+    w.MM = T(1) - ei - ed;
+    w.MI = ei;
+    w.MD = ed;
+    w.GM = T(1) - ec;
+    w.II = ec;
+    w.DD = ec;
+    w.pm = T(1) - eb;
+    w.px = eb / T(3);
+    w.base = g.rd_bases[ro + p];
+    w.n_read = (w.base == 'N');
+    return w;
+}
+
+template <typename T, int kCols>
+struct GradLds {
+    uint8_t *hap;
+    T *bm, *bi, *bs;  // forward boundary / relay row: M, I, S by column
+    T *bx, *bg;       // adjoint boundary: X and gI of the row below a stripe, by column
+};
+
+template <int kCols>
+constexpr int grad_stride() { return kCols + 1; }
+
+template <typename T, int kCols>
+constexpr size_t grad_lds_per_wave() {
+    return static_cast<size_t>(kCols) + 5 * static_cast<size_t>(grad_stride<kCols>()) * sizeof(T);
+}
+
+template <typename T, int kCols>
+__device__ __forceinline__ GradLds<T, kCols> grad_carve(uint8_t *base) {
+    GradLds<T, kCols> w;
+    w.hap = base;
+    w.bm = reinterpret_cast<T *>(base + kCols);
+    w.bi = w.bm + grad_stride<kCols>();
+    w.bs = w.bi + grad_stride<kCols>();
+    w.bx = w.bs + grad_stride<kCols>();
+    w.bg = w.bx + grad_stride<kCols>();
+    return w;
+}
+
+// One pair: forward with checkpoints, then the adjoint.  Returns nothing; writes the outputs.
+template <typename T, int kCols, int kTile, bool kHapHasN>
+__device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__restrict__ scratch, int lane,
+                          uint32_t pair, int H, int R, int64_t ro, double weight, const RescueSink &rs) {
+    const int n_stripes = (R + kRows - 1) / kRows;
+    const int steps = round_up(H + kWave, kTile);
+    const int n_tiles = steps / kTile;
+    T *ck = scratch;                                                        // [stripe][tile][6][64]
+    T *relay = scratch + static_cast<size_t>(g.max_stripes) * g.max_tiles * 6 * kWave;  // [stripe][3][kCols]
+    const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
+    const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
+
+    // ---- 1. forward, top down --------------------------------------------------------------
+    for (int s = 0; s < n_stripes; ++s) {
+        if (s > 0) {
+            T *dst = relay + static_cast<size_t>(s) * 3 * kCols;
+            for (int j = lane; j < kCols; j += kWave) {
+                dst[j] = L.bm[j];
+                dst[kCols + j] = L.bi[j];
+                dst[2 * kCols + j] = L.bs[j];
+            }
+        }
+        const Row<T> w = load_row<T>(g, ro, R, s * kRows + lane - 1, lane > 0);
+        const T px = w.n_read ? w.pm : w.px;
+        const int writer = (s == n_stripes - 1) ? (R - s * kRows) : kRows;
+        T m1 = 0, i1 = 0, d1 = 0, s1 = 0, m2 = 0, s2 = 0;
+        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
+        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *wsp = L.bs + kLdsPad - writer;
+        T *ckp = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
+        for (int t0 = 0; t0 < steps; t0 += kTile) {
+            ckp[0 * kWave] = m1;
+            ckp[1 * kWave] = i1;
+            ckp[2 * kWave] = d1;
+            ckp[3 * kWave] = s1;
+            ckp[4 * kWave] = m2;
+            ckp[5 * kWave] = s2;
+            ckp += 6 * kWave;
+#pragma unroll
+            for (int u = 0; u < kTile; ++u) {
+                const int t = t0 + u;
+                const uint32_t hb = hp[t];
+                const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
+                const T prior = match ? w.pm : px;
+                T a = from_prev_lane(m2) * w.MM;
+                a = fma(from_prev_lane(s2), w.GM, a);
+                T m = a * prior;
+                T i = from_prev_lane(m1) * w.MI;
+                i = fma(from_prev_lane(i1), w.II, i);
+                T d = m1 * w.MD;
+                d = fma(d1, w.DD, d);
+                T sx = i + d;
+                if (lane == writer) {
+                    wm[t] = m;
+                    wi[t] = i;
+                    wsp[t] = sx;
+                }
+                if (lane == 0) {
+                    m = um[t];
+                    i = ui[t];
+                    sx = us[t];
+                }
+                m2 = m1;
+                s2 = s1;
+                m1 = m;
+                i1 = i;
+                d1 = d;
+                s1 = sx;
+            }
+        }
+    }
+    double acc = 0.0;
+    for (int c = 1 + lane; c <= H; c += kWave)
+        acc += static_cast<double>(L.bm[kLdsPad + c]) + static_cast<double>(L.bi[kLdsPad + c]);
+    const double l_scaled = wave_sum(acc);
+    const double scale_log10 = static_cast<double>(Scale<T>::kExp) * 0.30102999566398119521;
+
+    // ---- scaling of the adjoint ----------------------------------------------------------------
+    int sigma = 0;
+    if (AdjointScale<T>::kMaxSigma > 0) {
+        sigma = AdjointScale<T>::kTarget - ilogb(l_scaled);
+        const int bad = __builtin_amdgcn_readfirstlane(static_cast<int>(!(l_scaled > 0.0) || sigma > AdjointScale<T>::kMaxSigma));
+        if (bad) {  // hand the pair to the FP64 kernel
+            if (rs.list != nullptr) {
+                const uint32_t slot = __builtin_amdgcn_readfirstlane(atomicAdd(rs.count, lane == 0 ? 1u : 0u));
+                rs.list[slot] = pair;
+            }
+            return;
+        }
+    }
+    g.out_log10[pair] = log10(l_scaled) - scale_log10;
+    if (!(l_scaled > 0.0)) return;  // FP64 underflow: log10 L = -inf, no gradient
+    const T seed = static_cast<T>(__builtin_ldexp(1.0, sigma));
+    const double norm = weight / (l_scaled * __builtin_ldexp(1.0, sigma) * 2.30258509299404568402);
+
+    // Adjoint boundary below the last stripe: the seed row (gI = seed on columns 1..H, X = 0).
+    for (int j = lane; j < kCols; j += kWave) {
+        const int c = j - kLdsPad;
+        L.bx[j] = 0;
+        L.bg[j] = (c >= 1 && c <= H) ? seed : T(0);
+    }
+
+    // ---- 2. adjoint, bottom up -------------------------------------------------------------------
+    for (int s = n_stripes - 1; s >= 0; --s) {
+        const int p = s * kRows + lane - 1;
+        const Row<T> w = load_row<T>(g, ro, R, p, lane > 0);
+        const T px = w.n_read ? w.pm : w.px;
+        // Next row's MM, GM, MI, II: lane k+1's own; lane 63's is the first row of the stripe below
+        // (or the seed row after the last stripe).
+        Row<T> below{T(0), T(0), T(1), T(1), T(0), T(0), T(0), T(0), 0u, false};
+        if (s < n_stripes - 1) below = load_row<T>(g, ro, R, (s + 1) * kRows, true);
+        const T MMn = wave_shl1(below.MM, w.MM), GMn = wave_shl1(below.GM, w.GM);
+        const T MIn = wave_shl1(below.MI, w.MI), IIn = wave_shl1(below.II, w.II);
+        // Relay row of this stripe into LDS (row 0 for the first stripe).
+        {
+            const T *src = relay + static_cast<size_t>(s) * 3 * kCols;
+            for (int j = lane; j < kCols; j += kWave) {
+                if (s == 0) {
+                    L.bm[j] = 0;
+                    L.bi[j] = 0;
+                    L.bs[j] = d0;
+                } else {
+                    L.bm[j] = src[j];
+                    L.bi[j] = src[kCols + j];
+                    L.bs[j] = src[2 * kCols + j];
+                }
+            }
+        }
+        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
+        const T *bxr = L.bx + kLdsPad - 62, *bgr = L.bg + kLdsPad - 63;
+        T *bxw = L.bx + kLdsPad - 1, *bgw = L.bg + kLdsPad - 1;
+        T X1 = 0, X2 = 0, gI1 = 0, gD1 = 0;
+        T aMMn = 0, aGMn = 0, aMIn = 0, aIIn = 0, aMD = 0, aDD = 0, aE = 0;
+        const T *ckbase = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
+        for (int tile = n_tiles - 1; tile >= 0; --tile) {
+            const int t0 = tile * kTile;
+            const T *ckp = ckbase + static_cast<size_t>(tile) * 6 * kWave;
+            T m1 = ckp[0 * kWave], i1 = ckp[1 * kWave], d1 = ckp[2 * kWave], s1 = ckp[3 * kWave];
+            T m2 = ckp[4 * kWave], s2 = ckp[5 * kWave];
+            const T m_before = m1, d_before = d1;
+            T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stP[kTile], stAW[kTile];
+#pragma unroll
+            for (int u = 0; u < kTile; ++u) {
+                const int t = t0 + u;
+                const uint32_t hb = hp[t];
+                const bool hit = (hb == w.base) || (kHapHasN && hb == 'N');
+                const T prior = hit ? w.pm : px;
+                const T wgt = (hit || w.n_read) ? T(-1) : T(1) / T(3);  // d prior / d e(qb)
+                T a = from_prev_lane(m2) * w.MM;
+                a = fma(from_prev_lane(s2), w.GM, a);
+                T m = a * prior;
+                T i = from_prev_lane(m1) * w.MI;
+                i = fma(from_prev_lane(i1), w.II, i);
+                T d = m1 * w.MD;
+                d = fma(d1, w.DD, d);
+                T sx = i + d;
+                if (lane == 0) {
+                    m = um[t];
+                    i = ui[t];
+                    sx = us[t];
+                }
+                stM[u] = m;
+                stI[u] = i;
+                stD[u] = d;
+                stS[u] = sx;
+                stP[u] = prior;
+                stAW[u] = a * wgt;
+                m2 = m1;
+                s2 = s1;
+                m1 = m;
+                i1 = i;
+                d1 = d;
+                s1 = sx;
+            }
+#pragma unroll
+            for (int u = kTile - 1; u >= 0; --u) {
+                const int t = t0 + u;
+                const T Xn2 = wave_shl1(bxr[t], X2);   // X(r+1, c+1)
+                const T gIn1 = wave_shl1(bgr[t], gI1); // gI(r+1, c)
+                const T gS = GMn * Xn2;
+                const T gD = fma(w.DD, gD1, gS);
+                const T gI = fma(IIn, gIn1, gS);
+                const T gM = fma(w.MD, gD1, fma(MIn, gIn1, MMn * Xn2));
+                const T X = stP[u] * gM;
+                const T Mp = (u > 0) ? stM[u - 1] : m_before;
+                const T Dp = (u > 0) ? stD[u - 1] : d_before;
+                aMMn = fma(Xn2, stM[u], aMMn);
+                aGMn = fma(Xn2, stS[u], aGMn);
+                aMIn = fma(gIn1, stM[u], aMIn);
+                aIIn = fma(gIn1, stI[u], aIIn);
+                aMD = fma(gD, Mp, aMD);
+                aDD = fma(gD, Dp, aDD);
+                aE = fma(gM, stAW[u], aE);
+                if (lane == 1) {
+                    bxw[t] = X;
+                    bgw[t] = gI;
+                }
+                X2 = X1;
+                X1 = X;
+                gI1 = gI;
+                gD1 = gD;
+            }
+        }
+        // The next-row sums belong to the lane below: shift them down one lane.
+        const double gMM = static_cast<double>(wave_shr1(T(0), aMMn));
+        const double gGM = static_cast<double>(wave_shr1(T(0), aGMn));
+        const double gMI = static_cast<double>(wave_shr1(T(0), aMIn));
+        const double gII = static_cast<double>(wave_shr1(T(0), aIIn));
+        if (lane > 0 && p < R) {
+            const double ei = static_cast<double>(w.MI), ed = static_cast<double>(w.MD), ec = static_cast<double>(w.II);
+            const double eb = 1.0 - static_cast<double>(w.pm);
+            const double dq = -0.23025850929940456840;  // d e(q)/dq = dq * e(q)
+            const double d_ei = (-gMM + gMI) * norm, d_ed = (-gMM + static_cast<double>(aMD)) * norm;
+            const double d_ec = (-gGM + gII + static_cast<double>(aDD)) * norm;
+            const double d_eb = static_cast<double>(aE) * norm;
+            atomicAdd(&g.g_qi[ro + p], d_ei * dq * ei);
+            atomicAdd(&g.g_qd[ro + p], d_ed * dq * ed);
+            atomicAdd(&g.g_qc[ro + p], d_ec * dq * ec);
+            atomicAdd(&g.g_qb[ro + p], d_eb * dq * eb);
+            if (g.g_eb) atomicAdd(&g.g_eb[ro + p], d_eb);
+        }
+    }
+}
+
+template <typename T, int kCols, int kTile>
+__device__ void grad_process(const GradBatch &g, const RescueSink &rs, uint8_t *lds, T *scratch, int lane, uint32_t pair) {
+    const int r = g.pair_read[pair], h = g.pair_hap[pair];
+    const int H = g.hap_len[h], R = g.rd_len[r];
+    const int64_t ro = g.rd_off[r];
+    const uint8_t *hap = g.hap_bases + g.hap_off[h];
+    const double weight = g.pair_weight ? g.pair_weight[pair] : 1.0;
+    GradLds<T, kCols> L = grad_carve<T, kCols>(lds);
+    // Guards: the scratch and LDS were sized from the caller's max_read_len / max_hap_len; a pair
+    // outside those bounds (or with an empty haplotype) gets NaN instead of touching memory.
+    const int n_stripes = (R + kRows - 1) / kRows;
+    const int n_tiles = round_up(H + kWave, kTile) / kTile;
+    if (H <= 0 || H + 2 * kLdsPad + 32 > kCols || n_stripes > g.max_stripes || n_tiles > g.max_tiles) {
+        g.out_log10[pair] = __builtin_nan("");
+        return;
+    }
+    int has_n = 0;
+    for (int j = lane; j < kCols; j += kWave) {
+        const int c = j - kLdsPad + 1;
+        uint8_t v = 0;
+        if (c >= 1 && c <= H) {
+            v = hap[c - 1];
+            has_n |= (v == 'N');
+        }
+        L.hap[j] = v;
+    }
+    const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
+    for (int j = lane; j < kCols; j += kWave) {
+        L.bm[j] = 0;
+        L.bi[j] = 0;
+        L.bs[j] = d0;
+    }
+    if (R == 0) {  // empty read: log10 L = -inf, nothing to differentiate (LoglessPairHMM.java:62-67)
+        g.out_log10[pair] = -__builtin_inf();
+        return;
+    }
+    if (__any(has_n))
+        grad_pair<T, kCols, kTile, true>(g, L, scratch, lane, pair, H, R, ro, weight, rs);
+    else
+        grad_pair<T, kCols, kTile, false>(g, L, scratch, lane, pair, H, R, ro, weight, rs);
+}
+
+template <typename T, int kCols, int kTile>
+__global__ __launch_bounds__(256) void grad_kernel(GradBatch g, WorkSource ws, RescueSink rs) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int waves_per_block = blockDim.x / kWave;
+    uint8_t *lds = lds_raw + grad_lds_per_wave<T, kCols>() * wave;
+    const size_t gwave = static_cast<size_t>(blockIdx.x) * waves_per_block + wave;
+    T *scratch = reinterpret_cast<T *>(g.scratch) + gwave * g.scratch_per_wave;
+    const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
+    for (;;) {
+        const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
+        if (it >= n_items) break;
+        grad_process<T, kCols, kTile>(g, rs, lds, scratch, lane, ws.items ? ws.items[it] : it);
+    }
+}
+
+}  // namespace
+
+size_t grad_lds_bytes_per_wave(int max_hap_len, bool fp64) {
+    const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
+    return cols + 5 * (cols + 1) * (fp64 ? sizeof(double) : sizeof(float));
+}
+
+size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_stripes, int *max_tiles) {
+    const int cols = lds_columns(max_hap_len);
+    *max_stripes = (max_read_len + kRows - 1) / kRows;
+    if (*max_stripes < 1) *max_stripes = 1;
+    *max_tiles = dev::round_up(max_hap_len + kWave, kGradTile) / kGradTile;
+    return static_cast<size_t>(*max_stripes) * (static_cast<size_t>(*max_tiles) * 6 * kWave + 3 * static_cast<size_t>(cols));
+}
+
+namespace {
+
+template <typename F>
+hipError_t with_grad_kernel(bool fp64, int cols, F &&f) {
+    switch (cols) {
+#define GATK_PAIRHMM_GRAD_CASE(C)                                                                 \
+    case C:                                                                                       \
+        return fp64 ? f(grad_kernel<double, C, kGradTile>) : f(grad_kernel<float, C, kGradTile>);
+        GATK_PAIRHMM_GRAD_CASE(384)
+        GATK_PAIRHMM_GRAD_CASE(768)
+        GATK_PAIRHMM_GRAD_CASE(1664)
+        GATK_PAIRHMM_GRAD_CASE(2560)
+#undef GATK_PAIRHMM_GRAD_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
+    LaunchShape sh;
+    const int cols = lds_columns(max_hap_len);
+    if (cols == 0 || cols > 2560) return sh;
+    const size_t per_wave = grad_lds_bytes_per_wave(max_hap_len, fp64);
+    int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
+    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) --wpb;
+    if (per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) return sh;
+    sh.waves_per_block = wpb;
+    sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
+    int blocks = 0;
+    hipError_t e = with_grad_kernel(fp64, cols, [&](auto kernel) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * wpb, sh.lds_bytes);
+    });
+    sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
+    return sh;
+}
+
+hipError_t launch_grad(bool fp64, const GradBatch &g, const WorkSource &work, const RescueSink &rescue, int max_hap_len,
+                       const LaunchShape &shape, int grid_blocks, hipStream_t stream) {
+    if (shape.waves_per_block < 1 || grid_blocks < 1 || work.head == nullptr) return hipErrorInvalidValue;
+    const dim3 block(kWave * shape.waves_per_block);
+    return with_grad_kernel(fp64, lds_columns(max_hap_len), [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, g, work, rescue);
+        return hipGetLastError();
+    });
+}
+
+}  // namespace gatk_pairhmm

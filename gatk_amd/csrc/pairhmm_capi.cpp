@@ -99,6 +99,7 @@ struct pairhmm_context {
     double *d_mm_prob = nullptr;
     DevBuf stage_dev;    // packed inputs of one computeLikelihoods call + outputs + counters
     HostPinned stage_host;
+    DevBuf grad_scratch32, grad_scratch64, grad_work;  // forward_backward scratch (grow-only)
     int waves_per_block = 4;
 };
 
@@ -164,6 +165,9 @@ int32_t pairhmm_done(pairhmm_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->stage_dev.release();
     ctx->stage_host.release();
+    ctx->grad_scratch32.release();
+    ctx->grad_scratch64.release();
+    ctx->grad_work.release();
     if (ctx->d_qual_err) (void)hipFree(ctx->d_qual_err);
     if (ctx->d_mm_prob) (void)hipFree(ctx->d_mm_prob);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -538,5 +542,113 @@ extern "C" int32_t pairhmm_batch_free(pairhmm_batch *bt) {
     if (bt->graph) (void)hipGraphExecDestroy(bt->graph);
     if (bt->d_mem) (void)hipFree(bt->d_mem);
     delete bt;
+    return PAIRHMM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward + backward
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *d, pairhmm_run_stats *stats) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!d) return fail(PAIRHMM_ERR_BAD_ARG, "null descriptor");
+    if (d->n_pairs < 0 || d->n_reads < 0 || d->n_haps < 0 || d->read_bytes < 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "negative sizes");
+    if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "more than 2^32 pairs");
+    pairhmm_run_stats st{};
+    if (d->n_pairs == 0) {
+        if (stats) *stats = st;
+        return PAIRHMM_OK;
+    }
+    if (d->max_hap_len <= 0 || d->max_read_len < 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "max_hap_len must be > 0 and max_read_len >= 0");
+    if (!d->read_bases || !d->read_qb || !d->read_qi || !d->read_qd || !d->read_qc || !d->read_off || !d->read_len ||
+        !d->hap_bases || !d->hap_off || !d->hap_len || !d->pair_read || !d->pair_hap || !d->out_log10_likelihood ||
+        !d->grad_qb || !d->grad_qi || !d->grad_qd || !d->grad_qc)
+        return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    const LaunchShape sh32 = grad_shape(false, d->max_hap_len, 4);
+    const LaunchShape sh64 = grad_shape(true, d->max_hap_len, 4);
+    if (sh32.waves_per_block == 0 || sh64.waves_per_block == 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(d->max_hap_len) +
+                                             " bases exceeds the adjoint kernel's LDS budget");
+    int max_stripes = 0, max_tiles = 0;
+    const size_t per_wave = grad_scratch_elems_per_wave(d->max_read_len, d->max_hap_len, &max_stripes, &max_tiles);
+    const int cus = std::max(ctx->num_cus, 1);
+    const int grid32 = sh32.blocks_per_cu * cus, grid64 = sh64.blocks_per_cu * cus;
+    const size_t waves32 = static_cast<size_t>(grid32) * sh32.waves_per_block;
+    const size_t waves64 = static_cast<size_t>(grid64) * sh64.waves_per_block;
+    if (ctx->grad_scratch32.reserve(waves32 * per_wave * sizeof(float)) != hipSuccess ||
+        ctx->grad_scratch64.reserve(waves64 * per_wave * sizeof(double)) != hipSuccess ||
+        ctx->grad_work.reserve(64 + sizeof(uint32_t) * static_cast<size_t>(d->n_pairs)) != hipSuccess)
+        return fail(PAIRHMM_ERR_OOM, "adjoint scratch allocation failed");
+    uint32_t *counters = static_cast<uint32_t *>(ctx->grad_work.ptr);  // [0] head32 [1] rescued [2] head64
+    uint32_t *rescue = counters + 16;
+    GradBatch g{};
+    g.hap_bases = d->hap_bases;
+    g.hap_off = d->hap_off;
+    g.hap_len = d->hap_len;
+    g.rd_bases = d->read_bases;
+    g.rd_qb = d->read_qb;
+    g.rd_qi = d->read_qi;
+    g.rd_qd = d->read_qd;
+    g.rd_qc = d->read_qc;
+    g.rd_off = d->read_off;
+    g.rd_len = d->read_len;
+    g.pair_read = d->pair_read;
+    g.pair_hap = d->pair_hap;
+    g.pair_weight = d->pair_weight;
+    g.out_log10 = d->out_log10_likelihood;
+    g.g_qb = d->grad_qb;
+    g.g_qi = d->grad_qi;
+    g.g_qd = d->grad_qd;
+    g.g_qc = d->grad_qc;
+    g.g_eb = d->grad_base_error;
+    g.scratch_per_wave = per_wave;
+    g.max_stripes = max_stripes;
+    g.max_tiles = max_tiles;
+    hipStream_t s = ctx->stream;
+    hipEvent_t ev[3] = {};
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    const size_t gbytes = sizeof(double) * static_cast<size_t>(d->read_bytes);
+    HIP_TRY(hipMemsetAsync(d->grad_qb, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipMemsetAsync(d->grad_qi, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipMemsetAsync(d->grad_qd, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipMemsetAsync(d->grad_qc, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
+    if (d->grad_base_error) HIP_TRY(hipMemsetAsync(d->grad_base_error, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_zero_counters(counters, 16, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
+    RescueSink none{nullptr, nullptr, 0.0};
+    if (d->use_double) {
+        HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
+        g.scratch = ctx->grad_scratch64.ptr;
+        WorkSource all{nullptr, static_cast<uint32_t>(d->n_pairs), nullptr, counters + 2};
+        HIP_TRY(launch_grad(true, g, all, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
+    } else {
+        g.scratch = ctx->grad_scratch32.ptr;
+        WorkSource all{nullptr, static_cast<uint32_t>(d->n_pairs), nullptr, counters + 0};
+        RescueSink rs{rescue, counters + 1, 0.0};
+        HIP_TRY(launch_grad(false, g, all, rs, d->max_hap_len, sh32, grid32, s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
+        g.scratch = ctx->grad_scratch64.ptr;
+        WorkSource again{rescue, 0, counters + 1, counters + 2};
+        HIP_TRY(launch_grad(true, g, again, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
+    }
+    HIP_TRY(hipEventRecord(ev[2], s), PAIRHMM_ERR_KERNEL);
+    uint32_t host_counters[4] = {};
+    HIP_TRY(hipMemcpyAsync(host_counters, counters, sizeof(host_counters), hipMemcpyDeviceToHost, s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(s), PAIRHMM_ERR_KERNEL);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[0], ev[2]) == hipSuccess) st.total_ms = ms;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.fp32_kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
+    st.n_rescued = d->use_double ? d->n_pairs : host_counters[1];
+    if (stats) *stats = st;
     return PAIRHMM_OK;
 }

@@ -1,0 +1,79 @@
+// pairhmm_device.hpp -- device helpers shared by the gfx950 PairHMM kernels (forward and backward).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pairhmm_kernels.hpp"
+
+namespace gatk_pairhmm {
+namespace dev {
+
+// Fixed-point scale of the state matrices: D(0,.) = 2^kExp / H.  Every cell is <= 2^kExp (the
+// recurrences are sub-stochastic), so FP32 keeps headroom below FLT_MAX = 2^128.
+template <typename T>
+struct Scale;
+template <>
+struct Scale<float> {
+    static constexpr int kExp = 120;
+};
+template <>
+struct Scale<double> {
+    static constexpr int kExp = 1020;
+};
+
+constexpr int kDppWaveShr1 = 0x138;  // DPP_WF_SR1: lane i reads lane i-1; lane 0 has no source
+
+// Lane k receives `src` from lane k-1; lane 0 keeps `old`.
+__device__ __forceinline__ float wave_shr1(float old, float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, src), kDppWaveShr1, 0xF,
+                                                                 0xF, false));
+}
+__device__ __forceinline__ double wave_shr1(double old, double src) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
+    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), kDppWaveShr1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32), kDppWaveShr1,
+                                               0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__host__ __device__ constexpr int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+constexpr int kDppWaveShl1 = 0x130;  // DPP_WF_SL1: lane i reads lane i+1; lane 63 has no source
+
+// Lane k receives `src` from lane k+1; lane 63 keeps `old`.
+__device__ __forceinline__ float wave_shl1(float old, float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, src), kDppWaveShl1, 0xF,
+                                                                 0xF, false));
+}
+__device__ __forceinline__ double wave_shl1(double old, double src) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
+    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), kDppWaveShl1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32), kDppWaveShl1,
+                                               0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
+
+// Lane k reads `src` of lane k-1; lane 0 reads 0.  With a zero fill the compiler folds the move
+// into the consuming v_mul/v_fmac as a DPP operand (no separate v_mov_dpp).
+__device__ __forceinline__ float from_prev_lane(float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src), kDppWaveShr1, 0xF, 0xF,
+                                                              true));
+}
+__device__ __forceinline__ double from_prev_lane(double src) { return wave_shr1(0.0, src); }
+
+constexpr int kRows = kWave - 1;  // real rows per stripe: lane 0 relays the row above
+
+}  // namespace dev
+}  // namespace gatk_pairhmm

@@ -23,78 +23,61 @@
 //     cell), which keeps the inner loop free of masks.
 //   * The last-row sum, the unscaling and log10 are done in FP64.  FP32 launches use S = 120 and
 //     flag pairs whose scaled sum is below the underflow threshold for an FP64 recompute (S = 1020).
+#include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
 
 namespace gatk_pairhmm {
 namespace {
 
-template <typename T>
-struct Scale;
-template <>
-struct Scale<float> {
-    static constexpr int kExp = 120;
-};
-template <>
-struct Scale<double> {
-    static constexpr int kExp = 1020;
-};
+using namespace dev;
 
-constexpr int kDppWaveShr1 = 0x138;  // DPP_WF_SR1: lane i reads lane i-1; lane 0 has no source
-
-// Lane k receives `src` from lane k-1; lane 0 keeps `old`.
-__device__ __forceinline__ float wave_shr1(float old, float src) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
-                                                                 __builtin_bit_cast(int, src), kDppWaveShr1, 0xF,
-                                                                 0xF, false));
-}
-__device__ __forceinline__ double wave_shr1(double old, double src) {
-    const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
-    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), kDppWaveShr1, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32), kDppWaveShr1,
-                                               0xF, 0xF, false);
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
-                                          static_cast<uint32_t>(lo));
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
-
-__host__ __device__ constexpr int round_up(int x, int m) { return (x + m - 1) / m * m; }
-
-constexpr int kUnroll = 4;  // wave-steps per unrolled loop body
+constexpr int kUnroll = 8;  // wave-steps per unrolled loop body
 
 template <typename T>
 struct WaveLds {
-    uint8_t *hap;  // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
-    T *bm, *bi, *bd;  // [cols]: boundary row, column c at kLdsPad + c
+    uint8_t *hap;      // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
+    T *bm, *bi, *bs;   // [cols]: boundary row M, I and S = I + D, column c at kLdsPad + c
 };
 
-template <typename T>
-__device__ __forceinline__ WaveLds<T> carve(uint8_t *base, int cols) {
+// Boundary planes are kCols + 1 elements apart: an odd stride keeps the compiler on single
+// ds_read_b32 / ds_write_b32 with one shared base register and immediate offsets (a multiple of 64
+// dwords would turn them into ds_read2st64 pairs that need an extra address add per step).
+template <int kCols>
+constexpr int plane_stride() { return kCols + 1; }
+
+template <typename T, int kCols>
+constexpr size_t lds_per_wave() { return static_cast<size_t>(kCols) + 3 * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T); }
+
+template <typename T, int kCols>
+__device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
     WaveLds<T> w;
     w.hap = base;
-    T *p = reinterpret_cast<T *>(base + cols);
-    w.bm = p;
-    w.bi = p + cols;
-    w.bd = p + 2 * cols;
+    w.bm = reinterpret_cast<T *>(base + kCols);
+    w.bi = w.bm + plane_stride<kCols>();
+    w.bs = w.bi + plane_stride<kCols>();
     return w;
 }
 
-// One stripe-sequential pass over the pair.  Returns the scaled last-row sum (FP64, all lanes).
-template <typename T, bool kHapHasN>
+
+// One pass over a pair, stripe by stripe.  Returns the scaled last-row sum (FP64, every lane).
+//
+// Stripe s covers rows 63 s + 1 .. 63 s + 63 on lanes 1..63; lane 0 relays row 63 s, read from the
+// LDS boundary row, so every real row gets its upper neighbours from lane k-1 by DPP.  Per step
+// each lane keeps its own M, I, D from the previous step and M, S = I + D from two steps back:
+//   M(r,c) = prior * (M(r-1,c-1) MM + S(r-1,c-1) GM)     [lane k-1, two steps back]
+//   I(r,c) = M(r-1,c) MI + I(r-1,c) II                    [lane k-1, one step back]
+//   D(r,c) = M(r,c-1) MD + D(r,c-1) DD                    [own lane, one step back]
+template <typename T, int kCols, bool kHapHasN>
 __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
                                const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
                                const DeviceTables &tab) {
-    const int n_stripes = (R + kWave - 1) / kWave;
+    const int n_stripes = (R + kRows - 1) / kRows;
     for (int s = 0; s < n_stripes; ++s) {
-        const int p = s * kWave + lane;
+        const int p = s * kRows + lane - 1;  // read position of this lane's row
         T MM = 0, GM = 0, MI = 0, II = 0, MD = 0, DD = 0, pm = 0, px = 0;
         uint32_t base = 0;
-        if (p < R) {
+        if (lane > 0 && p < R) {
             base = rb[p];
             const int q = rq[p], qi = ri[p], qd = rd[p], qc = rg[p];
             const double eq = tab.qual_err[q], ei = tab.qual_err[qi], ed = tab.qual_err[qd], ec = tab.qual_err[qc];
@@ -109,40 +92,50 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
             px = static_cast<T>(eq / 3.0);
             if (base == 'N') px = pm;
         }
-        const bool last = (s == n_stripes - 1);
-        const int writer = last ? (R - 1 - s * kWave) : (kWave - 1);
-        T m1 = 0, i1 = 0, d1 = 0;  // own cell at step t-1 (column c-1)
-        T mu = 0, iu = 0, du = 0;  // row above at column c-1 (received at step t-1)
+        // Lane holding row R on the last stripe, else lane 63 (row 63 (s+1), the next relay row).
+        const int writer = (s == n_stripes - 1) ? (R - s * kRows) : kRows;
+        T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
+        T m2 = 0, s2 = 0;                  // own cell two steps back (column c-2)
         const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
-        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *wd = L.bd + kLdsPad - writer;
-        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *ud = L.bd + kLdsPad;
-        // Steps run to column H + 63 for lane 63, rounded up to the unroll width; the extra
-        // steps only touch columns > H.
+        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
+        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *ws = L.bs + kLdsPad - writer;
         const int steps = round_up(H + kWave, kUnroll);
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
+            uint32_t hbs[kUnroll];  // haplotype bytes of the block, read ahead of their use
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const int t = t0 + u;
-                const T nmu = wave_shr1(um[t], m1);
-                const T niu = wave_shr1(ui[t], i1);
-                const T ndu = wave_shr1(ud[t], d1);
-                const uint32_t hb = hp[t];
+                const uint32_t hb = hbs[u];
                 const bool match = (hb == base) || (kHapHasN && hb == 'N');
                 const T prior = match ? pm : px;
-                const T m = prior * (mu * MM + (iu + du) * GM);
-                const T i = nmu * MI + niu * II;
-                const T d = m1 * MD + d1 * DD;
-                mu = nmu;
-                iu = niu;
-                du = ndu;
-                m1 = m;
-                i1 = i;
-                d1 = d;
+                T a = from_prev_lane(m2) * MM;
+                a = fma(from_prev_lane(s2), GM, a);
+                T m = a * prior;
+                T i = from_prev_lane(m1) * MI;
+                i = fma(from_prev_lane(i1), II, i);
+                T d = m1 * MD;
+                d = fma(d1, DD, d);
+                T sx = i + d;
+                // The writer stores before lane 0's relay load is issued: the load writes the same
+                // registers, so storing after it would wait for the LDS round trip every step.
                 if (lane == writer) {
                     wm[t] = m;
                     wi[t] = i;
-                    wd[t] = d;
+                    ws[t] = sx;
                 }
+                if (lane == 0) {
+                    m = um[t];
+                    i = ui[t];
+                    sx = us[t];
+                }
+                m2 = m1;
+                s2 = s1;
+                m1 = m;
+                i1 = i;
+                d1 = d;
+                s1 = sx;
             }
         }
     }
@@ -152,9 +145,9 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
     return wave_sum(acc);
 }
 
-template <typename T>
+template <typename T, int kCols>
 __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, const RescueSink &rs, uint8_t *lds,
-                             int cols, int lane, uint32_t pair) {
+                             int lane, uint32_t pair) {
     int r, h;
     if (b.pair_read) {
         r = b.pair_read[pair];
@@ -167,12 +160,12 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
     const int R = b.rd_len[r];
     const uint8_t *hap = b.hap_bases + b.hap_off[h];
     const int64_t ro = b.rd_off[r];
-    WaveLds<T> L = carve<T>(lds, cols);
+    WaveLds<T> L = carve<T, kCols>(lds);
 
     // Stage the haplotype (zero padded) and the row-0 boundary: M = I = 0, D = 2^S / H
     // (LoglessPairHMM.java:30-36).
     int has_n = 0;
-    for (int j = lane; j < cols; j += kWave) {
+    for (int j = lane; j < kCols; j += kWave) {
         const int c = j - kLdsPad + 1;  // hap byte j belongs to column c
         uint8_t v = 0;
         if (c >= 1 && c <= H) {
@@ -182,17 +175,17 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
         L.hap[j] = v;
     }
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
-    for (int j = lane; j < cols; j += kWave) {
+    for (int j = lane; j < kCols; j += kWave) {
         L.bm[j] = 0;
         L.bi[j] = 0;
-        L.bd[j] = d0;
+        L.bs[j] = d0;
     }
     double sum;
     if (__any(has_n))
-        sum = forward_pair<T, true>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+        sum = forward_pair<T, kCols, true>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
                                     b.rd_gcp + ro, tab);
     else
-        sum = forward_pair<T, false>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+        sum = forward_pair<T, kCols, false>(L, lane, H, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
                                      b.rd_gcp + ro, tab);
     // Every lane holds the same sum (butterfly reduction), so the result and the rescue decision are
     // wave-uniform; all lanes store the same value to the same address rather than branching on
@@ -208,15 +201,14 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTables tab, WorkSource ws, RescueSink rs,
-                                                      int cols) {
+template <typename T, int kCols>
+__global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTables tab, WorkSource ws, RescueSink rs) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
-    const size_t per_wave = static_cast<size_t>(cols) + 3 * static_cast<size_t>(cols) * sizeof(T);
-    uint8_t *lds = lds_raw + per_wave * wave;
+    constexpr size_t kPerWave = lds_per_wave<T, kCols>();
+    uint8_t *lds = lds_raw + kPerWave * wave;
     const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
     if (ws.head) {
         for (;;) {
@@ -224,12 +216,12 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
             // lane-divergent branch; lane 0's pre-increment value is the wave's item.
             const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
             if (it >= n_items) break;
-            process_pair<T>(b, tab, rs, lds, cols, lane, ws.items ? ws.items[it] : it);
+            process_pair<T, kCols>(b, tab, rs, lds, lane, ws.items ? ws.items[it] : it);
         }
     } else {
         const uint32_t stride = gridDim.x * waves_per_block;
         for (uint32_t it = blockIdx.x * waves_per_block + wave; it < n_items; it += stride)
-            process_pair<T>(b, tab, rs, lds, cols, lane, ws.items ? ws.items[it] : it);
+            process_pair<T, kCols>(b, tab, rs, lds, lane, ws.items ? ws.items[it] : it);
     }
 }
 
@@ -239,28 +231,53 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
 
 }  // namespace
 
-// Columns -64..H+63+kUnroll of the boundary row and haplotype, plus slack, rounded to 16.
-int lds_columns(int max_hap_len) { return round_up(max_hap_len + 2 * kLdsPad + 16, 16); }
+// Column capacity classes of the LDS slice (haplotype bytes + three boundary planes).  The class is
+// a template parameter so every LDS access is one base register plus an immediate offset.  A class
+// holds columns -64..H+63+kUnroll plus slack.
+constexpr int kColClasses[] = {384, 768, 1664, 2560, 4992};
+
+int lds_columns(int max_hap_len) {
+    const int need = max_hap_len + 2 * kLdsPad + 32;
+    for (int c : kColClasses)
+        if (c >= need) return c;
+    return 0;
+}
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + 3 * cols * (fp64 ? sizeof(double) : sizeof(float));
+    return cols + 3 * (cols + 1) * (fp64 ? sizeof(double) : sizeof(float));
 }
+
+namespace {
+
+template <typename F>
+hipError_t with_kernel(bool fp64, int cols, F &&f) {
+    switch (cols) {
+#define GATK_PAIRHMM_CASE(C)                                                    \
+    case C:                                                                     \
+        return fp64 ? f(forward_kernel<double, C>) : f(forward_kernel<float, C>);
+        GATK_PAIRHMM_CASE(384)
+        GATK_PAIRHMM_CASE(768)
+        GATK_PAIRHMM_CASE(1664)
+        GATK_PAIRHMM_CASE(2560)
+        GATK_PAIRHMM_CASE(4992)
+#undef GATK_PAIRHMM_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
 
 hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTables &tables, const WorkSource &work,
                           const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,
                           hipStream_t stream) {
     if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
-    const int cols = lds_columns(max_hap_len);
     const dim3 block(kWave * shape.waves_per_block);
-    if (fp64) {
-        hipLaunchKernelGGL(forward_kernel<double>, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables,
-                           work, rescue, cols);
-    } else {
-        hipLaunchKernelGGL(forward_kernel<float>, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables,
-                           work, rescue, cols);
-    }
-    return hipGetLastError();
+    return with_kernel(fp64, lds_columns(max_hap_len), [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables, work, rescue);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream) {
@@ -270,6 +287,8 @@ hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream) {
 
 LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
     LaunchShape sh;
+    const int cols = lds_columns(max_hap_len);
+    if (cols == 0) return sh;
     const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64);
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
     while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) --wpb;
@@ -277,11 +296,9 @@ LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_bl
     sh.waves_per_block = wpb;
     sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
     int blocks = 0;
-    hipError_t e;
-    if (fp64)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, forward_kernel<double>, kWave * wpb, sh.lds_bytes);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, forward_kernel<float>, kWave * wpb, sh.lds_bytes);
+    hipError_t e = with_kernel(fp64, cols, [&](auto kernel) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * wpb, sh.lds_bytes);
+    });
     sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
     return sh;
 }

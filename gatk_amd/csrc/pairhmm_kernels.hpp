@@ -70,4 +70,34 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
 // Zeroes n counters with a kernel (graph-capturable; no memset node).
 hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream);
 
+// ---- forward + backward (adjoint) over continuous Phred parameters --------------------------
+
+constexpr int kGradTile = 8;  // wave-steps per recompute tile of the adjoint
+
+// Device pointers of a gradient batch.  Read parameters are float Phred values per read position.
+struct GradBatch {
+    const uint8_t *hap_bases;
+    const int64_t *hap_off;
+    const int32_t *hap_len;
+    const uint8_t *rd_bases;
+    const float *rd_qb, *rd_qi, *rd_qd, *rd_qc;
+    const int64_t *rd_off;
+    const int32_t *rd_len;
+    const int32_t *pair_read;
+    const int32_t *pair_hap;
+    const double *pair_weight;  // may be null (weight 1)
+    double *out_log10;          // [n_pairs]
+    double *g_qb, *g_qi, *g_qd, *g_qc;  // [read bytes], accumulated
+    double *g_eb;               // [read bytes] or null
+    void *scratch;              // per-wave checkpoints and relay rows
+    size_t scratch_per_wave;    // elements (float or double) per wave
+    int max_stripes, max_tiles;
+};
+
+size_t grad_lds_bytes_per_wave(int max_hap_len, bool fp64);
+size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_stripes, int *max_tiles);
+LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block);
+hipError_t launch_grad(bool fp64, const GradBatch &g, const WorkSource &work, const RescueSink &rescue, int max_hap_len,
+                       const LaunchShape &shape, int grid_blocks, hipStream_t stream);
+
 }  // namespace gatk_pairhmm

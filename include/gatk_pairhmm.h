@@ -141,6 +141,36 @@ int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *batch, int32_
 
 int32_t pairhmm_batch_free(pairhmm_batch *batch);
 
+/* ---- forward + backward (new; no counterpart in the reference) ------------------------------ */
+
+/* Differentiable forward over continuous per-position Phred parameters.  With e(q) = 10^(-q/10):
+ * MM = 1 - e(qi) - e(qd), MI = e(qi), MD = e(qd), GM = 1 - e(qc), II = DD = e(qc), prior =
+ * 1 - e(qb) on a match and e(qb)/3 otherwise; the recurrence is LoglessPairHMM's
+ * (LoglessPairHMM.java:47-57).  All pointers are DEVICE pointers on the context's device. */
+typedef struct pairhmm_grad_desc {
+    int64_t n_reads, n_haps, n_pairs;
+    int64_t read_bytes;
+    const uint8_t *read_bases;
+    const float *read_qb, *read_qi, *read_qd, *read_qc; /* Phred per read position (read_bytes) */
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *hap_bases;
+    const int64_t *hap_off;
+    const int32_t *hap_len;
+    const int32_t *pair_read, *pair_hap;
+    const double *pair_weight;        /* w_p, may be NULL (all 1)                              */
+    int32_t max_read_len, max_hap_len; /* bounds over the batch (size the scratch and LDS)     */
+    int32_t use_double;               /* nonzero: every pair through the FP64 kernel            */
+    double *out_log10_likelihood;     /* [n_pairs]                                             */
+    double *grad_qb, *grad_qi, *grad_qd, *grad_qc; /* [read_bytes]: sum_p w_p d log10 L_p / d q */
+    double *grad_base_error;          /* [read_bytes] or NULL: sum_p w_p d log10 L_p / d e(qb)  */
+} pairhmm_grad_desc;
+
+/* Runs the forward pass and the adjoint over every pair; overwrites the outputs.  FP32 cells with
+ * FP32 adjoint sums and FP64 per-position reductions; pairs whose scaled likelihood is too small for
+ * the FP32 adjoint range are recomputed in FP64.  Synchronous. */
+int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *desc, pairhmm_run_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
