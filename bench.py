@@ -55,6 +55,92 @@ def cpu_baseline(batch, out, seconds: float, threads: int):
                        f"(oracle/pairhmm_oracle.c), OpenMP over pairs, {t_used:.1f} s"), worst, done
 
 
+def fwd_bwd_leg(args):
+    """Forward + backward GCUPS on a batched_uneven batch and forward GCUPS on the same pairs."""
+    import torch
+
+    from gatk_amd import _native, synth
+    from gatk_amd.autograd import GradEngine, GradProblem
+    from gatk_amd.batch import BatchEngine
+
+    gb = synth.workload("batched_uneven", n_regions=args.bwd_regions, seed=3)
+    cells = gb.cells()
+    prob = GradProblem.from_batch(gb)
+    n = len(gb.read_bases)
+    qb = torch.from_numpy(gb.read_quals.astype(np.float32)).cuda()
+    qi = torch.full((n,), 45.0, dtype=torch.float32, device="cuda")
+    qd = torch.full((n,), 45.0, dtype=torch.float32, device="cuda")
+    qc = torch.full((n,), 10.0, dtype=torch.float32, device="cuda")
+    with GradEngine() as eng:
+        out, grads = eng.run(prob, qb, qi, qd, qc)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(args.bwd_steps):
+            out, grads = eng.run(prob, qb, qi, qd, qc)
+        el = time.perf_counter() - t0
+        st = eng.last_stats
+    gcups = cells * args.bwd_steps / el / 1e9
+    with BatchEngine() as beng:
+        dev = beng.upload(gb)
+        dev.forward()
+        t0 = time.perf_counter()
+        for _ in range(args.bwd_steps):
+            dev.forward()
+        fel = time.perf_counter() - t0
+        dev.free()
+    fwd = cells * args.bwd_steps / fel / 1e9
+    log(f"fwd+bwd: {gcups:.1f} GCUPS ({el / args.bwd_steps * 1e3:.0f} ms/step, fp32 kernel {st.fp32_kernel_ms:.0f} ms, "
+        f"fp64 {st.rescue_kernel_ms:.0f} ms, {st.n_rescued} pairs in FP64); forward on the same pairs {fwd:.1f} GCUPS")
+    res = {"gcups": round(gcups, 2), "fwd_gcups_same_pairs": round(fwd, 2), "ratio_to_fwd": round(gcups / fwd, 3),
+           "workload": f"batched_uneven {args.bwd_regions} regions (seed 3), qb = base quals, qi = qd = 45, qc = 10",
+           "pairs": gb.n_pairs, "cells": cells, "fp64_pairs": int(st.n_rescued)}
+    ctx = dict(batch=gb, out=out.cpu().numpy(), grads={k: v.cpu().numpy() for k, v in grads.items()},
+               q=(gb.read_quals.astype(np.float32), np.full(n, 45.0, np.float32), np.full(n, 45.0, np.float32),
+                  np.full(n, 10.0, np.float32)))
+    return res, ctx
+
+
+def cpu_grad_baseline(ctx, threads, n_reads=2):
+    """FP64 adjoint port (full matrices) on every pair of a few sampled reads: CPU fwd+bwd GCUPS and
+    the accuracy of the GPU's per-read gradient sums and log10 L for those reads."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle
+
+    b = ctx["batch"]
+    rng = np.random.default_rng(7)
+    reads = rng.choice(len(b.read_len), size=n_reads, replace=False)
+    pairs = np.nonzero(np.isin(b.pair_read, reads))[0]
+
+    def one(p):
+        r, h = int(b.pair_read[p]), int(b.pair_hap[p])
+        o, R = int(b.read_off[r]), int(b.read_len[r])
+        ho, H = int(b.hap_off[h]), int(b.hap_len[h])
+        qs = [x[o:o + R].astype(np.float64) for x in ctx["q"]]
+        return oracle.logless_float_grad(b.hap_bases[ho:ho + H], b.read_bases[o:o + R], *qs), R * H
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, pairs))
+    el = time.perf_counter() - t0
+    cells = sum(c for _, c in res)
+    worst_l = max(abs(ctx["out"][p] - g["log10_l"]) for p, (g, _) in zip(pairs, res))
+    worst_g = 0.0
+    for r in reads:
+        o, R = int(b.read_off[r]), int(b.read_len[r])
+        want = {k: np.zeros(R) for k in ("qb", "qi", "qd", "qc")}
+        for p, (g, _) in zip(pairs, res):
+            if b.pair_read[p] == r:
+                for k in want:
+                    want[k] += g[k]
+        a = np.concatenate([ctx["grads"][k][o:o + R] for k in want])
+        w = np.concatenate([want[k] for k in want])
+        worst_g = max(worst_g, float(np.linalg.norm(a - w) / max(np.linalg.norm(w), 1e-300)))
+    return {"cpu_fwd_bwd_baseline": {"value": cells / el / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
+                                     "sample": f"{len(pairs)} pairs of {n_reads} random reads, FP64 adjoint port "
+                                               f"(oracle/pairhmm_oracle.c, full matrices), {el:.1f} s"},
+            "max_abs_dlog10L_sample": worst_l, "grad_rel_err_sample": worst_g}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +150,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-regions", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bwd-regions", type=int, default=2_000,
+                    help="regions of the forward+backward leg (0 skips it)")
+    ap.add_argument("--bwd-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -146,11 +235,17 @@ def main() -> int:
         log(f"plain per-region drop-in: {plain:.1f} GCUPS over {len(prepared)} regions")
     eng.close()
 
+    # Forward + backward (adjoint) leg: d log10 L / d(per-position Phred qb, qi, qd, qc).
+    bwd = None
+    if args.bwd_regions > 0:
+        bwd, bwd_ctx = fwd_bwd_leg(args)
     cpu, worst, n_checked = None, None, 0
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu, worst, n_checked = cpu_baseline(batch, out, args.cpu_seconds, threads)
         log(f"cpu baseline {cpu['value']:.3f} GCUPS on {threads} threads; max |dlog10L| on {n_checked} pairs = {worst:.2e}")
+        if bwd is not None:
+            bwd.update(cpu_grad_baseline(bwd_ctx, threads))
 
     line = {
         "metric": "PairHMM GCUPS fwd & fwd+bwd on 1 MI355X at |dlog10L|<=1e-4 vs FP64",
@@ -169,7 +264,8 @@ def main() -> int:
                                "Poisson(80) reads of 100-250 bp, hipGraph persistent kernel",
                    "regions": batch.n_regions, "pairs": batch.n_pairs, "cells": cells,
                    "rescued_pairs": rescued},
-        "fwd_bwd_gcups": None,
+        "fwd_bwd_gcups": bwd["gcups"] if bwd else None,
+        "fwd_bwd": bwd,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
         "max_abs_dlog10L_vs_fp64_sample": worst,

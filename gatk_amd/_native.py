@@ -34,6 +34,12 @@ EXPORTED_SYMBOLS = (
     "pairhmm_batch_forward",
     "pairhmm_batch_free",
     "pairhmm_forward_backward",
+    "pairhmm_set_tuning",
+    "pairhmm_get_tuning",
+    "pairhmm_stream_create",
+    "pairhmm_stream_submit",
+    "pairhmm_stream_finish",
+    "pairhmm_stream_destroy",
 )
 
 
@@ -113,6 +119,11 @@ class GradDesc(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_block", ctypes.c_int32), ("grid_percent", ctypes.c_int32),
+                ("queue_order", ctypes.c_int32)]
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double),
@@ -133,6 +144,13 @@ def load_library() -> ctypes.CDLL:
     with _lock:
         if _lib is not None:
             return _lib
+        # PyTorch's ROCm wheel bundles its own HIP runtime (torch/lib/libamdhip64.so).  If torch is
+        # present it is imported first, so that the library binds to that same runtime instead of
+        # loading a second HIP runtime into the process (torch then fails to see the GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise NativeLibraryMissing(
                 f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
@@ -157,6 +175,18 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_batch_free.restype = i32
         L.pairhmm_forward_backward.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
         L.pairhmm_forward_backward.restype = i32
+        L.pairhmm_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        L.pairhmm_set_tuning.restype = i32
+        L.pairhmm_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        L.pairhmm_get_tuning.restype = i32
+        L.pairhmm_stream_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.pairhmm_stream_create.restype = i32
+        L.pairhmm_stream_submit.argtypes = [vp, ctypes.POINTER(BatchDesc), vp]
+        L.pairhmm_stream_submit.restype = i32
+        L.pairhmm_stream_finish.argtypes = [vp, ctypes.POINTER(RunStats)]
+        L.pairhmm_stream_finish.restype = i32
+        L.pairhmm_stream_destroy.argtypes = [vp]
+        L.pairhmm_stream_destroy.restype = i32
         _lib = L
         return L
 

@@ -1,0 +1,104 @@
+"""Autotuner of the batched forward launch schedule.
+
+The persistent forward kernel has three schedule knobs (``pairhmm_tuning`` in
+include/gatk_pairhmm.h): wavefronts per workgroup (which sets how the per-wave LDS slices pack into
+a CU's 160 KB), the persistent grid as a fraction of resident capacity, and the work-queue order
+(descending cost vs submission order).  ``tune`` uploads a sample of the workload, times every
+candidate's FP32 kernel with HIP events (eager launches, best of ``repeats``), and returns the
+fastest.  Results are cached in a JSON file keyed by device and a coarse workload signature
+(haplotype / read length classes), so later runs on the same device and workload shape reuse them.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import json
+import os
+from dataclasses import asdict, dataclass
+from typing import Dict, Iterable, List, Optional
+
+import numpy as np
+
+from . import _native
+from .batch import BatchEngine, PackedBatch
+from .pairhmm import _raise_for
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "gatk_amd", "autotune.json")
+
+
+@dataclass(frozen=True)
+class Schedule:
+    waves_per_block: int = 4
+    grid_percent: int = 100
+    queue_order: int = 0
+
+    def apply(self, engine: BatchEngine) -> None:
+        t = _native.Tuning(self.waves_per_block, self.grid_percent, self.queue_order)
+        _raise_for(engine.lib.pairhmm_set_tuning(engine.ctx, ctypes.byref(t)), "set_tuning")
+
+
+CANDIDATES: List[Schedule] = [Schedule(w, g, o) for w, g, o in
+                              itertools.product((1, 2, 4, 8), (50, 100, 200), (0, 1))]
+
+
+def workload_signature(batch: PackedBatch) -> str:
+    """Coarse shape of a workload: 64-column haplotype class of the 95th percentile and 63-row
+    stripe count of the 95th-percentile read."""
+    h95 = int(np.percentile(batch.hap_len, 95))
+    r95 = int(np.percentile(batch.read_len, 95)) if len(batch.read_len) else 0
+    return f"h{(h95 + 63) // 64 * 64}-s{(r95 + 62) // 63}"
+
+
+def device_name() -> str:
+    try:
+        import torch
+
+        return torch.cuda.get_device_name(0)
+    except Exception:  # torch absent: one key per host
+        return "gfx950"
+
+
+def _load(path: str) -> Dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(batch: PackedBatch, candidates: Iterable[Schedule] = CANDIDATES, sample_regions: int = 300,
+         repeats: int = 2, cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True,
+         verbose: bool = False) -> Schedule:
+    """Pick the fastest schedule for this workload shape (see module docstring)."""
+    key = f"{device_name()}|{workload_signature(batch)}"
+    if use_cache and cache_path:
+        hit = _load(cache_path).get(key)
+        if hit:
+            return Schedule(**hit["schedule"])
+    n = min(sample_regions, batch.n_regions)
+    sample = batch.subset_regions(range(n)) if n < batch.n_regions else batch
+    results = []
+    with BatchEngine() as eng:
+        for cand in candidates:
+            cand.apply(eng)
+            dev = eng.upload(sample)
+            best = float("inf")
+            for _ in range(repeats):
+                st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER)
+                best = min(best, st.fp32_kernel_ms)
+            dev.free()
+            results.append((best, cand))
+            if verbose:
+                print(f"[autotune] {cand}: {best:.2f} ms")
+    results.sort(key=lambda x: x[0])
+    best_ms, best = results[0]
+    if cache_path:
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        db = _load(cache_path)
+        db[key] = {"schedule": asdict(best), "fp32_kernel_ms": best_ms, "sample_regions": n,
+                   "ranking": [[ms, asdict(c)] for ms, c in results[:5]]}
+        tmp = cache_path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache_path)
+    return best

@@ -1,0 +1,127 @@
+// capi_internal.hpp -- internals shared by the C-ABI translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gatk_pairhmm.h"
+#include "model_tables.hpp"
+#include "pairhmm_kernels.hpp"
+
+namespace gatk_pairhmm {
+namespace capi {
+
+inline thread_local std::string g_last_error;
+
+inline int32_t fail(int32_t status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+
+#define HIP_TRY(expr, status)                                                                           \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail((status), std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Underflow threshold on the FP32 scaled last-row sum (scale 2^120): a pair whose sum is below it
+// is recomputed in FP64.  Below 1e-28 the dominant paths' partial sums approach the FP32 normal
+// range (1.2e-38) and lose relative precision.
+constexpr double kFp32RescueThreshold = 1e-28;
+
+inline bool device_is_gfx950(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 4096);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct HostPinned {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 4096);
+        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// PairHMM.computeReadLikelihoodGivenHaplotypeLog10 argument checks (PairHMM.java:284-292) and
+// PairHMMModel.qualToTransProbs sign checks (PairHMMModel.java:108-111).
+inline int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *ins, const uint8_t *del, const uint8_t *gcp,
+                      int64_t len, int64_t idx) {
+    if (len < 0) return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has negative length");
+    if (len == 0) return PAIRHMM_OK;
+    if (!b || !q || !ins || !del || !gcp)
+        return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has a null array");
+    for (int64_t i = 0; i < len; ++i) {
+        if (q[i] > kMaxQual)
+            return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + ": base quality 255 is out of range");
+        if ((ins[i] | del[i] | gcp[i]) & 0x80)
+            return fail(PAIRHMM_ERR_BAD_ARG,
+                        "read " + std::to_string(idx) + ": insertion/deletion/GCP quality cannot be negative");
+    }
+    return PAIRHMM_OK;
+}
+
+
+// Validates a host batch descriptor and builds its work queue (pair indices by descending
+// wave-step cost, or submission order).  Fills cells (sum of R*H) and the longest haplotype.
+int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
+                           int &max_hap_len);
+
+}  // namespace capi
+}  // namespace gatk_pairhmm
+
+// The opaque context of the C-ABI.
+struct pairhmm_context {
+    int device = 0;
+    int num_cus = 0;
+    bool use_double = false;
+    hipStream_t stream = nullptr;
+    double *d_qual_err = nullptr;
+    double *d_mm_prob = nullptr;
+    gatk_pairhmm::capi::DevBuf stage_dev;    // packed inputs of one computeLikelihoods call + outputs
+    gatk_pairhmm::capi::HostPinned stage_host;
+    gatk_pairhmm::capi::DevBuf grad_scratch32, grad_scratch64, grad_work;  // forward_backward scratch
+    int waves_per_block = 4;  // launch schedule (pairhmm_set_tuning)
+    int grid_percent = 100;
+    int queue_order = 0;
+};

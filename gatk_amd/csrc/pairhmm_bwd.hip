@@ -426,6 +426,8 @@ hipError_t with_grad_kernel(bool fp64, int cols, F &&f) {
         return fp64 ? f(grad_kernel<double, C, kGradTile>) : f(grad_kernel<float, C, kGradTile>);
         GATK_PAIRHMM_GRAD_CASE(384)
         GATK_PAIRHMM_GRAD_CASE(768)
+        GATK_PAIRHMM_GRAD_CASE(1024)
+        GATK_PAIRHMM_GRAD_CASE(1280)
         GATK_PAIRHMM_GRAD_CASE(1664)
         GATK_PAIRHMM_GRAD_CASE(2560)
 #undef GATK_PAIRHMM_GRAD_CASE
@@ -441,9 +443,10 @@ LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block
     const int cols = lds_columns(max_hap_len);
     if (cols == 0 || cols > 2560) return sh;
     const size_t per_wave = grad_lds_bytes_per_wave(max_hap_len, fp64);
+    const size_t budget = max_lds_per_block();
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
-    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) --wpb;
-    if (per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) return sh;
+    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > budget) --wpb;
+    if (per_wave * static_cast<size_t>(wpb) > budget) return sh;
     sh.waves_per_block = wpb;
     sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
     int blocks = 0;
@@ -459,6 +462,9 @@ hipError_t launch_grad(bool fp64, const GradBatch &g, const WorkSource &work, co
     if (shape.waves_per_block < 1 || grid_blocks < 1 || work.head == nullptr) return hipErrorInvalidValue;
     const dim3 block(kWave * shape.waves_per_block);
     return with_grad_kernel(fp64, lds_columns(max_hap_len), [&](auto kernel) {
+        if (shape.lds_bytes > 65536)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(shape.lds_bytes));
         hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, g, work, rescue);
         return hipGetLastError();
     });

@@ -12,96 +12,10 @@
 #include <string>
 #include <vector>
 
-#include "../../include/gatk_pairhmm.h"
-#include "model_tables.hpp"
-#include "pairhmm_kernels.hpp"
+#include "capi_internal.hpp"
 
 using namespace gatk_pairhmm;
-
-namespace {
-
-thread_local std::string g_last_error;
-
-int32_t fail(int32_t status, const std::string &msg) {
-    g_last_error = msg;
-    return status;
-}
-
-#define HIP_TRY(expr, status)                                                                           \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail((status), std::string(#expr) + ": " + hipGetErrorString(e_));                   \
-    } while (0)
-
-// Underflow threshold on the FP32 scaled last-row sum (scale 2^120): a pair whose sum is below it
-// is recomputed in FP64.  Below 1e-28 the dominant paths' partial sums approach the FP32 normal
-// range (1.2e-38) and lose relative precision.
-constexpr double kFp32RescueThreshold = 1e-28;
-
-bool device_is_gfx950(int dev) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
-    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
-}
-
-// Device buffer that only grows.
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    hipError_t reserve(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(n, 4096);
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-};
-
-struct HostPinned {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    hipError_t reserve(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (ptr) (void)hipHostFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(n, 4096);
-        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (ptr) (void)hipHostFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-};
-
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-}  // namespace
-
-struct pairhmm_context {
-    int device = 0;
-    int num_cus = 0;
-    bool use_double = false;
-    hipStream_t stream = nullptr;
-    double *d_qual_err = nullptr;
-    double *d_mm_prob = nullptr;
-    DevBuf stage_dev;    // packed inputs of one computeLikelihoods call + outputs + counters
-    HostPinned stage_host;
-    DevBuf grad_scratch32, grad_scratch64, grad_work;  // forward_backward scratch (grow-only)
-    int waves_per_block = 4;
-};
+using namespace gatk_pairhmm::capi;
 
 struct pairhmm_batch {
     pairhmm_context *ctx = nullptr;
@@ -176,28 +90,6 @@ int32_t pairhmm_done(pairhmm_context *ctx) {
 }
 
 }  // extern "C"
-
-namespace {
-
-// PairHMM.computeReadLikelihoodGivenHaplotypeLog10 argument checks (PairHMM.java:284-292) and
-// PairHMMModel.qualToTransProbs sign checks (PairHMMModel.java:108-111).
-int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *ins, const uint8_t *del, const uint8_t *gcp,
-                      int64_t len, int64_t idx) {
-    if (len < 0) return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has negative length");
-    if (len == 0) return PAIRHMM_OK;
-    if (!b || !q || !ins || !del || !gcp)
-        return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + " has a null array");
-    for (int64_t i = 0; i < len; ++i) {
-        if (q[i] > kMaxQual)
-            return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(idx) + ": base quality 255 is out of range");
-        if ((ins[i] | del[i] | gcp[i]) & 0x80)
-            return fail(PAIRHMM_ERR_BAD_ARG,
-                        "read " + std::to_string(idx) + ": insertion/deletion/GCP quality cannot be negative");
-    }
-    return PAIRHMM_OK;
-}
-
-}  // namespace
 
 extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
                                                int32_t n_reads, const pairhmm_haplotype_data_holder *haps,
@@ -323,17 +215,18 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
 // Batch API
 // ---------------------------------------------------------------------------------------------
 
-extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batch_desc *d, pairhmm_batch **out) {
-    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
-    if (!d || !out) return fail(PAIRHMM_ERR_BAD_ARG, "null argument");
-    *out = nullptr;
+namespace gatk_pairhmm {
+namespace capi {
+
+int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
+                           int &max_h) {
     if (d->n_reads < 0 || d->n_haps <= 0 || d->n_pairs < 0 || d->read_bytes < 0 || d->hap_bytes <= 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
     if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "batch exceeds 2^32 pairs");
     if (d->n_reads > 0 && (!d->read_off || !d->read_len)) return fail(PAIRHMM_ERR_BAD_ARG, "null read tables");
     if (!d->hap_bases || !d->hap_off || !d->hap_len) return fail(PAIRHMM_ERR_BAD_ARG, "null haplotype tables");
     if (d->n_pairs > 0 && (!d->pair_read || !d->pair_hap)) return fail(PAIRHMM_ERR_BAD_ARG, "null pair tables");
-    int max_h = 1;
+    max_h = 1;
     for (int64_t h = 0; h < d->n_haps; ++h) {
         if (d->hap_len[h] <= 0 || d->hap_off[h] < 0 || d->hap_off[h] + d->hap_len[h] > d->hap_bytes)
             return fail(PAIRHMM_ERR_BAD_ARG, "haplotype " + std::to_string(h) + " is empty or out of range");
@@ -352,26 +245,45 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     // Work queue: pairs sorted by descending cost (stripes x (H + 64) wave-steps) with a counting
     // sort, so long pairs start first and the tail of the persistent launch is short.
     const int64_t np = d->n_pairs;
-    std::vector<uint32_t> order(static_cast<size_t>(np));
-    int64_t cells = 0;
-    {
-        std::vector<uint32_t> cost(static_cast<size_t>(np));
-        uint32_t max_cost = 0;
-        for (int64_t p = 0; p < np; ++p) {
-            const int32_t r = d->pair_read[p], h = d->pair_hap[p];
-            if (r < 0 || r >= d->n_reads || h < 0 || h >= d->n_haps)
-                return fail(PAIRHMM_ERR_BAD_ARG, "pair " + std::to_string(p) + " indexes outside the batch");
-            const int R = d->read_len[r], H = d->hap_len[h];
-            cells += static_cast<int64_t>(R) * H;
-            const uint32_t c = static_cast<uint32_t>((R + kWave - 1) / kWave) * static_cast<uint32_t>(H + kWave);
-            cost[p] = c;
-            max_cost = std::max(max_cost, c);
-        }
+    order.resize(static_cast<size_t>(np));
+    cells = 0;
+    std::vector<uint32_t> cost(static_cast<size_t>(np));
+    uint32_t max_cost = 0;
+    for (int64_t p = 0; p < np; ++p) {
+        const int32_t r = d->pair_read[p], h = d->pair_hap[p];
+        if (r < 0 || r >= d->n_reads || h < 0 || h >= d->n_haps)
+            return fail(PAIRHMM_ERR_BAD_ARG, "pair " + std::to_string(p) + " indexes outside the batch");
+        const int R = d->read_len[r], H = d->hap_len[h];
+        cells += static_cast<int64_t>(R) * H;
+        // wave-steps of the pair: stripes of 63 rows, each H + 64 steps long
+        const uint32_t c = static_cast<uint32_t>((R + kWave - 2) / (kWave - 1)) * static_cast<uint32_t>(H + kWave);
+        cost[p] = c;
+        max_cost = std::max(max_cost, c);
+    }
+    if (queue_order == 1) {
+        for (int64_t p = 0; p < np; ++p) order[p] = static_cast<uint32_t>(p);
+    } else {
         std::vector<int64_t> start(static_cast<size_t>(max_cost) + 2, 0);
         for (int64_t p = 0; p < np; ++p) start[max_cost - cost[p] + 1]++;
         for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
         for (int64_t p = 0; p < np; ++p) order[start[max_cost - cost[p]]++] = static_cast<uint32_t>(p);
     }
+    return PAIRHMM_OK;
+}
+
+}  // namespace capi
+}  // namespace gatk_pairhmm
+
+extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batch_desc *d, pairhmm_batch **out) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!d || !out) return fail(PAIRHMM_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    std::vector<uint32_t> order;
+    int64_t cells = 0;
+    int max_h = 1;
+    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h);
+    if (vst != PAIRHMM_OK) return vst;
+    const int64_t np = d->n_pairs;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
@@ -437,8 +349,12 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     }
-    bt->fp32_grid = bt->sh32.blocks_per_cu * std::max(ctx->num_cus, 1);
-    bt->fp64_grid = bt->sh64.blocks_per_cu * std::max(ctx->num_cus, 1);
+    const auto scaled = [&](const LaunchShape &sh) {
+        const int64_t g = static_cast<int64_t>(sh.blocks_per_cu) * std::max(ctx->num_cus, 1) * ctx->grid_percent / 100;
+        return static_cast<int>(std::max<int64_t>(1, g));
+    };
+    bt->fp32_grid = scaled(bt->sh32);
+    bt->fp64_grid = scaled(bt->sh64);
     *out = bt;
     return PAIRHMM_OK;
 }
@@ -650,5 +566,28 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
     st.n_rescued = d->use_double ? d->n_pairs : host_counters[1];
     if (stats) *stats = st;
+    return PAIRHMM_OK;
+}
+
+extern "C" int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning *t) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!t) return fail(PAIRHMM_ERR_BAD_ARG, "null tuning");
+    const int wpb = t->waves_per_block == 0 ? 4 : t->waves_per_block;
+    const int gp = t->grid_percent == 0 ? 100 : t->grid_percent;
+    if (wpb < 1 || wpb > 8) return fail(PAIRHMM_ERR_BAD_ARG, "waves_per_block must be 1..8");
+    if (gp < 1 || gp > 400) return fail(PAIRHMM_ERR_BAD_ARG, "grid_percent must be 1..400");
+    if (t->queue_order != 0 && t->queue_order != 1) return fail(PAIRHMM_ERR_BAD_ARG, "queue_order must be 0 or 1");
+    ctx->waves_per_block = wpb;
+    ctx->grid_percent = gp;
+    ctx->queue_order = t->queue_order;
+    return PAIRHMM_OK;
+}
+
+extern "C" int32_t pairhmm_get_tuning(pairhmm_context *ctx, pairhmm_tuning *t) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!t) return fail(PAIRHMM_ERR_BAD_ARG, "null tuning");
+    t->waves_per_block = ctx->waves_per_block;
+    t->grid_percent = ctx->grid_percent;
+    t->queue_order = ctx->queue_order;
     return PAIRHMM_OK;
 }

@@ -234,7 +234,7 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
 // Column capacity classes of the LDS slice (haplotype bytes + three boundary planes).  The class is
 // a template parameter so every LDS access is one base register plus an immediate offset.  A class
 // holds columns -64..H+63+kUnroll plus slack.
-constexpr int kColClasses[] = {384, 768, 1664, 2560, 4992};
+constexpr int kColClasses[] = {384, 768, 1024, 1280, 1664, 2560, 4992};
 
 int lds_columns(int max_hap_len) {
     const int need = max_hap_len + 2 * kLdsPad + 32;
@@ -258,6 +258,8 @@ hipError_t with_kernel(bool fp64, int cols, F &&f) {
         return fp64 ? f(forward_kernel<double, C>) : f(forward_kernel<float, C>);
         GATK_PAIRHMM_CASE(384)
         GATK_PAIRHMM_CASE(768)
+        GATK_PAIRHMM_CASE(1024)
+        GATK_PAIRHMM_CASE(1280)
         GATK_PAIRHMM_CASE(1664)
         GATK_PAIRHMM_CASE(2560)
         GATK_PAIRHMM_CASE(4992)
@@ -275,6 +277,9 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
     if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
     const dim3 block(kWave * shape.waves_per_block);
     return with_kernel(fp64, lds_columns(max_hap_len), [&](auto kernel) {
+        if (shape.lds_bytes > 65536)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(shape.lds_bytes));
         hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables, work, rescue);
         return hipGetLastError();
     });
@@ -285,14 +290,27 @@ hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream) {
     return hipGetLastError();
 }
 
+size_t max_lds_per_block() {
+    static const size_t limit = [] {
+        int dev = 0, v = 0;
+        size_t l = 65536;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            l = static_cast<size_t>(v);
+        return l < 163840 ? l : static_cast<size_t>(163840);
+    }();
+    return limit;
+}
+
 LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
     LaunchShape sh;
     const int cols = lds_columns(max_hap_len);
     if (cols == 0) return sh;
     const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64);
+    const size_t budget = max_lds_per_block();
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
-    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) --wpb;
-    if (per_wave * static_cast<size_t>(wpb) > kMaxLdsPerBlock) return sh;
+    while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > budget) --wpb;
+    if (per_wave * static_cast<size_t>(wpb) > budget) return sh;
     sh.waves_per_block = wpb;
     sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
     int blocks = 0;

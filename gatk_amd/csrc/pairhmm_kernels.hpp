@@ -51,7 +51,8 @@ struct RescueSink {
 int lds_columns(int max_hap_len);
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64);
 
-constexpr size_t kMaxLdsPerBlock = 65536;  // dynamic LDS budget per workgroup
+// Dynamic LDS budget per workgroup: the device's limit (queried once), at most 160 KB (gfx950).
+size_t max_lds_per_block();
 
 // Launch geometry of the forward kernel for one batch.
 struct LaunchShape {

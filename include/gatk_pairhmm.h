@@ -141,6 +141,32 @@ int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *batch, int32_
 
 int32_t pairhmm_batch_free(pairhmm_batch *batch);
 
+/* Launch schedule of the batched forward (new).  Applied by later pairhmm_batch_upload calls on
+ * the context; gatk_amd/autotune.py searches it per device and workload. */
+typedef struct pairhmm_tuning {
+    int32_t waves_per_block; /* wavefronts per workgroup, 1..8 (0: 4)                          */
+    int32_t grid_percent;    /* persistent grid as a percentage of resident capacity (0: 100)  */
+    int32_t queue_order;     /* 0: pairs by descending cost (default), 1: submission order     */
+} pairhmm_tuning;
+
+int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning *tuning);
+int32_t pairhmm_get_tuning(pairhmm_context *ctx, pairhmm_tuning *tuning);
+
+/* ---- streaming (new; no counterpart in the reference) ------------------------------------- */
+
+/* Double-buffered streaming of region batches: pairhmm_stream_submit validates and packs a chunk
+ * (same layout as pairhmm_batch_desc, host memory) into pinned staging and returns while the device
+ * still computes the previous chunk; the H2D copy overlaps that compute.  The chunk's arrays may be
+ * released when submit returns; out_host (n_pairs doubles) must stay valid until it is filled,
+ * which happens when its slot is reused (two submits later) or at pairhmm_stream_finish. */
+typedef struct pairhmm_stream pairhmm_stream;
+
+int32_t pairhmm_stream_create(pairhmm_context *ctx, pairhmm_stream **out_stream);
+int32_t pairhmm_stream_submit(pairhmm_stream *stream, const pairhmm_batch_desc *chunk, double *out_host);
+/* Waits for every submitted chunk; stats: host wall time since the first submit, cells, rescued. */
+int32_t pairhmm_stream_finish(pairhmm_stream *stream, pairhmm_run_stats *stats);
+int32_t pairhmm_stream_destroy(pairhmm_stream *stream);
+
 /* ---- forward + backward (new; no counterpart in the reference) ------------------------------ */
 
 /* Differentiable forward over continuous per-position Phred parameters.  With e(q) = 10^(-q/10):

@@ -187,4 +187,5 @@ def test_gradcheck_fp64_kernel():
         f = lambda *a: pairhmm_log10_likelihood(prob, *a, eng)  # noqa: E731
         # Phred inputs reach the kernel as float32, so the step is 1e-2 (a 1e-3 step would be
         # resolved to only ~1e-3 relative at Q ~ 30).
-        assert torch.autograd.gradcheck(f, tq, eps=1e-2, atol=1e-5, rtol=1e-2, fast_mode=True)
+        # FP64 atomics accumulate positions in a run-dependent order: allow last-bit differences.
+        assert torch.autograd.gradcheck(f, tq, eps=1e-2, atol=1e-5, rtol=1e-2, fast_mode=True, nondet_tol=1e-10)
