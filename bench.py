@@ -141,6 +141,23 @@ def cpu_grad_baseline(ctx, threads, n_reads=2):
             "max_abs_dlog10L_sample": worst_l, "grad_rel_err_sample": worst_g}
 
 
+def str_fit_leg(args):
+    """Per-(STR period, repeats) GOP / GCP fit with the adjoint kernel against known penalties."""
+    from gatk_amd import strfit
+
+    spec = strfit.FitSpec(n_regions=args.fit_regions)
+    t0 = time.perf_counter()
+    wl, res, rep = strfit.run_gpu_fit(spec, max_iter=args.fit_iters, log=log)
+    total = time.perf_counter() - t0
+    b = wl.batch
+    d = {"regions": spec.n_regions, "reads": int(len(b.read_len)), "cells_per_evaluation": res.cells_per_eval,
+         "evaluations": res.evaluations, "fit_seconds": round(res.seconds, 2), "total_seconds": round(total, 2),
+         "fwd_bwd_gcups_in_fit": round(res.cells_per_eval * res.evaluations / res.seconds / 1e9, 1),
+         "final_neg_mean_log10L": round(res.loss, 6), "recovered": rep.ok, **rep.as_dict()}
+    log(f"STR fit: {d}")
+    return d
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +170,10 @@ def main() -> int:
     ap.add_argument("--bwd-regions", type=int, default=2_000,
                     help="regions of the forward+backward leg (0 skips it)")
     ap.add_argument("--bwd-steps", type=int, default=2)
+    ap.add_argument("--fit-regions", type=int, default=2_000,
+                    help="regions of the STR penalty fit leg (100 reads each; 0 skips it)")
+    ap.add_argument("--fit-iters", type=int, default=60)
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-schedule autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -183,6 +204,14 @@ def main() -> int:
         f"{time.perf_counter() - t0:.1f} s")
 
     eng = BatchEngine()
+    schedule = None
+    if not args.no_tune:
+        from gatk_amd import autotune
+
+        t0 = time.perf_counter()
+        schedule = autotune.tune(batch, use_cache=False, cache_path=None)
+        schedule.apply(eng)
+        log(f"autotuner picked {schedule} in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     dev = eng.upload(batch)
     log(f"uploaded to HBM in {time.perf_counter() - t0:.1f} s")
@@ -239,6 +268,9 @@ def main() -> int:
     bwd = None
     if args.bwd_regions > 0:
         bwd, bwd_ctx = fwd_bwd_leg(args)
+    fit = None
+    if args.fit_regions > 0:
+        fit = str_fit_leg(args)
     cpu, worst, n_checked = None, None, 0
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -266,6 +298,8 @@ def main() -> int:
                    "rescued_pairs": rescued},
         "fwd_bwd_gcups": bwd["gcups"] if bwd else None,
         "fwd_bwd": bwd,
+        "str_fit": fit,
+        "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
         "max_abs_dlog10L_vs_fp64_sample": worst,

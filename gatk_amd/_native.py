@@ -40,6 +40,9 @@ EXPORTED_SYMBOLS = (
     "pairhmm_stream_submit",
     "pairhmm_stream_finish",
     "pairhmm_stream_destroy",
+    "pairhmm_dragstr_analyze",
+    "pairhmm_dragstr_contexts",
+    "pairhmm_dragstr_impute",
 )
 
 
@@ -187,6 +190,13 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_stream_finish.restype = i32
         L.pairhmm_stream_destroy.argtypes = [vp]
         L.pairhmm_stream_destroy.restype = i32
+        i64 = ctypes.c_int64
+        L.pairhmm_dragstr_analyze.argtypes = [vp, i32, i32, vp, vp]
+        L.pairhmm_dragstr_analyze.restype = i32
+        L.pairhmm_dragstr_contexts.argtypes = [vp, vp, vp, i64, i64, i32, i32, vp]
+        L.pairhmm_dragstr_contexts.restype = i32
+        L.pairhmm_dragstr_impute.argtypes = [vp, vp, i32, i32, vp, vp, vp, i64, i64, vp, vp]
+        L.pairhmm_dragstr_impute.restype = i32
         _lib = L
         return L
 

@@ -197,6 +197,31 @@ typedef struct pairhmm_grad_desc {
  * the FP32 adjoint range are recomputed in FP64.  Synchronous. */
 int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *desc, pairhmm_run_stats *stats);
 
+/* ---- DRAGstr input side (host code; no device needed) -------------------------------------- */
+
+/* Per read position: the period (1..max_period) with the most tandem copies overlapping the
+ * position or starting right after it, shorter period on ties, and that copy count.
+ * Replaces DragstrReadSTRAnalyzer.of(bases, maxPeriod).mostRepeatedPeriod(i) /
+ * numberOfMostRepeats(i) (DragstrReadSTRAnalyzer.java:69-139). */
+int32_t pairhmm_dragstr_analyze(const uint8_t *bases, int32_t length, int32_t max_period, int32_t *out_period,
+                                int32_t *out_repeats);
+
+/* Context index per read position of a packed read set: (min(period, P) - 1) * R + min(repeats, R) - 1
+ * for every position but the last of each read, -1 for the last (whose penalties are fixed).  The
+ * index is the cell DragstrParams.gop/gcp(period, repeats) reads (DragstrParams.java:194-200). */
+int32_t pairhmm_dragstr_contexts(const uint8_t *read_bases, const int64_t *read_off, const int32_t *read_len,
+                                 int64_t n_reads, int64_t read_bytes, int32_t max_period, int32_t max_repeats,
+                                 int32_t *out_context);
+
+/* DragstrPairHMMInputScoreImputator.impute for a packed read set
+ * (DragstrPairHMMInputScoreImputator.java:53-70): GOP = round(min(40, gop[period][repeats])), used
+ * for insertions and deletions alike, GCP = round(gcp[period][repeats]), last position 45 / 10.
+ * Tables are row-major [max_period][max_repeats] Phred values. */
+int32_t pairhmm_dragstr_impute(const double *gop_table, const double *gcp_table, int32_t max_period,
+                               int32_t max_repeats, const uint8_t *read_bases, const int64_t *read_off,
+                               const int32_t *read_len, int64_t n_reads, int64_t read_bytes, uint8_t *out_gop,
+                               uint8_t *out_gcp);
+
 #ifdef __cplusplus
 }
 #endif
