@@ -102,10 +102,23 @@ inline int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *
 }
 
 
-// Validates a host batch descriptor and builds its work queue (pair indices by descending
-// wave-step cost, or submission order).  Fills cells (sum of R*H) and the longest haplotype.
+// A contiguous run of the work queue launched as one persistent FP32 kernel with the LDS slice of
+// its longest haplotype.  Queue order 0 groups pairs by LDS column class (largest first) and sorts
+// each group by descending cost, so short haplotypes run at the occupancy their smaller slice
+// allows; queue order 1 is one run in submission order.
+struct QueueSlice {
+    int64_t begin = 0, count = 0;
+    int max_hap_len = 1;
+};
+constexpr int kCounterSlots = 16;  // [0] head of slice 0, [1] rescue count, [2] rescue head, [2+k] head of slice k
+constexpr int kMaxSlices = kCounterSlots - 2;
+
+inline uint32_t *slice_head(uint32_t *counters, size_t k) { return counters + (k == 0 ? 0 : 2 + k); }
+
+// Validates a host batch descriptor and builds its work queue and slices.  Fills cells (sum of
+// R*H) and the longest haplotype.
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_hap_len);
+                           int &max_hap_len, std::vector<QueueSlice> &slices);
 
 }  // namespace capi
 }  // namespace gatk_pairhmm
@@ -125,3 +138,19 @@ struct pairhmm_context {
     int grid_percent = 100;
     int queue_order = 0;
 };
+
+namespace gatk_pairhmm {
+namespace capi {
+
+// Persistent grid for a launch shape: resident blocks x CUs x the tuning percentage, and no more
+// blocks than the queue has work for.
+int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_t n_items);
+
+// Enqueues the FP32 forward kernel once per queue slice on `s` (counters must be zeroed first);
+// pairs below the rescue threshold are appended to rescue_list.
+hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
+                              const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
+                              uint32_t *rescue_list, hipStream_t s);
+
+}  // namespace capi
+}  // namespace gatk_pairhmm

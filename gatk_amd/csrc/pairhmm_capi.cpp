@@ -26,10 +26,11 @@ struct pairhmm_batch {
     DeviceBatch view{};
     uint32_t *d_order = nullptr;    // work queue: pair indices by descending cost
     uint32_t *d_rescue = nullptr;   // rescue list (n_pairs capacity)
-    uint32_t *d_counters = nullptr; // [0] queue head, [1] rescue count
+    uint32_t *d_counters = nullptr; // kCounterSlots: queue heads and rescue counters
     hipGraphExec_t graph = nullptr;
-    LaunchShape sh32, sh64;
-    int fp32_grid = 0, fp64_grid = 0;
+    std::vector<QueueSlice> slices; // FP32 launches over the queue
+    LaunchShape sh64;               // FP64 rescue launch
+    int fp64_grid = 0;
 };
 
 extern "C" {
@@ -219,7 +220,7 @@ namespace gatk_pairhmm {
 namespace capi {
 
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_h) {
+                           int &max_h, std::vector<QueueSlice> &slices) {
     if (d->n_reads < 0 || d->n_haps <= 0 || d->n_pairs < 0 || d->read_bytes < 0 || d->hap_bytes <= 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
     if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "batch exceeds 2^32 pairs");
@@ -242,10 +243,9 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         int32_t st = validate_read(d->read_bases, d->read_quals, d->read_ins, d->read_del, d->read_gcp, d->read_bytes, -1);
         if (st != PAIRHMM_OK) return st;
     }
-    // Work queue: pairs sorted by descending cost (stripes x (H + 64) wave-steps) with a counting
-    // sort, so long pairs start first and the tail of the persistent launch is short.
     const int64_t np = d->n_pairs;
     order.resize(static_cast<size_t>(np));
+    slices.clear();
     cells = 0;
     std::vector<uint32_t> cost(static_cast<size_t>(np));
     uint32_t max_cost = 0;
@@ -260,15 +260,101 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         cost[p] = c;
         max_cost = std::max(max_cost, c);
     }
+    if (np == 0) return PAIRHMM_OK;
     if (queue_order == 1) {
         for (int64_t p = 0; p < np; ++p) order[p] = static_cast<uint32_t>(p);
-    } else {
+        slices.push_back(QueueSlice{0, np, max_h});
+        return PAIRHMM_OK;
+    }
+    if (lds_class_index(max_h) < 0)
+        return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
+    // Counting sort by descending cost, then a stable counting sort by descending LDS class.
+    std::vector<uint32_t> by_cost(static_cast<size_t>(np));
+    {
         std::vector<int64_t> start(static_cast<size_t>(max_cost) + 2, 0);
         for (int64_t p = 0; p < np; ++p) start[max_cost - cost[p] + 1]++;
         for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
-        for (int64_t p = 0; p < np; ++p) order[start[max_cost - cost[p]]++] = static_cast<uint32_t>(p);
+        for (int64_t p = 0; p < np; ++p) by_cost[start[max_cost - cost[p]]++] = static_cast<uint32_t>(p);
+    }
+    const int top = lds_class_index(max_h);
+    std::vector<int64_t> n_in(static_cast<size_t>(top) + 1, 0);
+    std::vector<int> h_in(static_cast<size_t>(top) + 1, 1);
+    for (int64_t p = 0; p < np; ++p) {
+        const int H = d->hap_len[d->pair_hap[p]];
+        const int k = lds_class_index(H);
+        n_in[k]++;
+        h_in[k] = std::max(h_in[k], H);
+    }
+    // Classes with few pairs are merged into the next larger one: a launch per class costs a drain
+    // of the persistent grid, worth it only for a class with real work.  At most kMaxSlices runs.
+    std::vector<int> run_of(static_cast<size_t>(top) + 1, 0);
+    std::vector<int64_t> run_start;
+    std::vector<int> run_class;
+    {
+        int64_t carried = 0;
+        for (int k = 0; k <= top; ++k) {
+            carried += n_in[k];
+            const bool close = (k == top) || carried >= std::max<int64_t>(4096, np / 64);
+            run_of[k] = static_cast<int>(run_class.size());
+            if (close && carried > 0) {
+                run_class.push_back(k);
+                carried = 0;
+            }
+        }
+        // classes merged into a later run take that run's index
+        for (int k = top; k >= 0; --k)
+            if (run_of[k] >= static_cast<int>(run_class.size())) run_of[k] = static_cast<int>(run_class.size()) - 1;
+        while (static_cast<int>(run_class.size()) > kMaxSlices) {
+            // fold the smallest run into its larger neighbour
+            for (int k = 0; k <= top; ++k)
+                if (run_of[k] == 0) run_of[k] = 1;
+            run_class.erase(run_class.begin());
+            for (int k = 0; k <= top; ++k) run_of[k] -= 1;
+        }
+    }
+    const int n_runs = static_cast<int>(run_class.size());
+    std::vector<int64_t> run_n(static_cast<size_t>(n_runs), 0);
+    std::vector<int> run_h(static_cast<size_t>(n_runs), 1);
+    for (int k = 0; k <= top; ++k) {
+        run_n[run_of[k]] += n_in[k];
+        if (n_in[k] > 0) run_h[run_of[k]] = std::max(run_h[run_of[k]], h_in[k]);
+    }
+    // Largest class first: runs in descending order of their class.
+    std::vector<int64_t> pos(static_cast<size_t>(n_runs), 0);
+    int64_t acc = 0;
+    for (int r = n_runs - 1; r >= 0; --r) {
+        pos[r] = acc;
+        if (run_n[r] > 0) slices.push_back(QueueSlice{acc, run_n[r], run_h[r]});
+        acc += run_n[r];
+    }
+    for (int64_t i = 0; i < np; ++i) {
+        const uint32_t p = by_cost[i];
+        const int r = run_of[lds_class_index(d->hap_len[d->pair_hap[p]])];
+        order[pos[r]++] = p;
     }
     return PAIRHMM_OK;
+}
+
+int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_t n_items) {
+    const int64_t g = static_cast<int64_t>(shape.blocks_per_cu) * std::max(ctx->num_cus, 1) * ctx->grid_percent / 100;
+    const int64_t need = (std::max<int64_t>(n_items, 1) + shape.waves_per_block - 1) / std::max(shape.waves_per_block, 1);
+    return static_cast<int>(std::max<int64_t>(1, std::min(g, need)));
+}
+
+hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
+                              const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
+                              uint32_t *rescue_list, hipStream_t s) {
+    RescueSink rs{rescue_list, counters + 1, kFp32RescueThreshold};
+    for (size_t k = 0; k < slices.size(); ++k) {
+        const QueueSlice &q = slices[k];
+        const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block);
+        if (sh.waves_per_block == 0) return hipErrorInvalidValue;
+        WorkSource w{order + q.begin, static_cast<uint32_t>(q.count), nullptr, slice_head(counters, k)};
+        const hipError_t e = launch_forward(false, batch, tab, w, rs, q.max_hap_len, sh,
+                                            persistent_grid(ctx, sh, q.count), s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace capi
@@ -279,9 +365,10 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     if (!d || !out) return fail(PAIRHMM_ERR_BAD_ARG, "null argument");
     *out = nullptr;
     std::vector<uint32_t> order;
+    std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h);
+    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices);
     if (vst != PAIRHMM_OK) return vst;
     const int64_t np = d->n_pairs;
     size_t off = 0;
@@ -294,7 +381,7 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     const size_t o_rb = take(rb), o_rq = take(rb), o_ri = take(rb), o_rd = take(rb), o_rg = take(rb), o_hb = take(hb),
                  o_roff = take(8 * d->n_reads), o_rlen = take(4 * d->n_reads), o_hoff = take(8 * d->n_haps),
                  o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np),
-                 o_out = take(8 * np), o_res = take(4 * np), o_cnt = take(4 * 16);
+                 o_out = take(8 * np), o_res = take(4 * np), o_cnt = take(4 * kCounterSlots);
     HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
     pairhmm_batch *bt = new (std::nothrow) pairhmm_batch();
     if (!bt) return fail(PAIRHMM_ERR_OOM, "batch allocation failed");
@@ -343,18 +430,13 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     bt->d_order = reinterpret_cast<uint32_t *>(m + o_ord);
     bt->d_rescue = reinterpret_cast<uint32_t *>(m + o_res);
     bt->d_counters = reinterpret_cast<uint32_t *>(m + o_cnt);
-    bt->sh32 = forward_shape(false, max_h, ctx->waves_per_block);
+    bt->slices = std::move(slices);
     bt->sh64 = forward_shape(true, max_h, ctx->waves_per_block);
-    if (bt->sh32.waves_per_block == 0 || bt->sh64.waves_per_block == 0) {
+    if (forward_shape(false, max_h, ctx->waves_per_block).waves_per_block == 0 || bt->sh64.waves_per_block == 0) {
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     }
-    const auto scaled = [&](const LaunchShape &sh) {
-        const int64_t g = static_cast<int64_t>(sh.blocks_per_cu) * std::max(ctx->num_cus, 1) * ctx->grid_percent / 100;
-        return static_cast<int>(std::max<int64_t>(1, g));
-    };
-    bt->fp32_grid = scaled(bt->sh32);
-    bt->fp64_grid = scaled(bt->sh64);
+    bt->fp64_grid = persistent_grid(ctx, bt->sh64, np);
     *out = bt;
     return PAIRHMM_OK;
 }
@@ -366,12 +448,10 @@ namespace {
 hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
     pairhmm_context *ctx = bt->ctx;
     DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
-    hipError_t e = launch_zero_counters(bt->d_counters, 16, s);
+    hipError_t e = launch_zero_counters(bt->d_counters, kCounterSlots, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[0], s);
-    WorkSource q{bt->d_order, static_cast<uint32_t>(bt->n_pairs), nullptr, bt->d_counters + 0};
-    RescueSink rs{bt->d_rescue, bt->d_counters + 1, kFp32RescueThreshold};
-    e = launch_forward(false, bt->view, tab, q, rs, bt->max_hap_len, bt->sh32, bt->fp32_grid, s);
+    e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
     WorkSource q2{bt->d_rescue, 0, bt->d_counters + 1, bt->d_counters + 2};
@@ -424,7 +504,7 @@ extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt
         HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
     } else if (mode == PAIRHMM_RUN_FP64) {
         DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
-        HIP_TRY(launch_zero_counters(bt->d_counters, 16, s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(launch_zero_counters(bt->d_counters, kCounterSlots, s), PAIRHMM_ERR_KERNEL);
         HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
         WorkSource q{bt->d_order, static_cast<uint32_t>(bt->n_pairs), nullptr, bt->d_counters + 0};
         RescueSink none{nullptr, nullptr, 0.0};

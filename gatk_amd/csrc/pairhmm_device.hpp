@@ -73,6 +73,21 @@ __device__ __forceinline__ float from_prev_lane(float src) {
 }
 __device__ __forceinline__ double from_prev_lane(double src) { return wave_shr1(0.0, src); }
 
+// a += src_a(lane k-1) * mul_a and b += src_b(lane k-1) * mul_b, lane 0 reading 0 for both sources:
+// two v_fmac_f32 with a DPP wave_shr:1 source operand.  The compiler folds a DPP move into v_mul but
+// not into v_fmac (tied accumulator), which would leave one v_mov_b32_dpp per term; writing the
+// instructions out saves both.  The compiler's hazard recognizer does not look inside inline
+// assembly, so the leading s_nop 4 provides the wait states a DPP read needs after a VALU write of
+// its source (2) and after an EXEC write (5).
+__device__ __forceinline__ void fmac_prev_lane2(float &a, float src_a, float mul_a, float &b, float src_b,
+                                                float mul_b) {
+    asm("s_nop 4\n\t"
+        "v_fmac_f32_dpp %0, %2, %3 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %1, %4, %5 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+&v"(a), "+&v"(b)
+        : "v"(src_a), "v"(mul_a), "v"(src_b), "v"(mul_b));
+}
+
 constexpr int kRows = kWave - 1;  // real rows per stripe: lane 0 relays the row above
 
 }  // namespace dev

@@ -10,15 +10,16 @@
 // (PairHMMModel.java:169-171); priors follow LoglessPairHMM.java:84-92 with tristate correction.
 //
 // Mapping onto a CDNA wavefront (one read/haplotype pair per wave):
-//   * Rows are processed in stripes of 64: lane k owns read row 64 s + k + 1 of stripe s and keeps
-//     that row's six transition probabilities and two priors in registers.
+//   * Rows are processed in stripes of 63: lanes 1..63 own read rows 63 s + 1 .. 63 s + 63 of stripe
+//     s and keep their row's six transition probabilities and two priors in registers; lane 0
+//     relays row 63 s, loaded from an LDS boundary row.
 //   * At step t lane k computes column c = t - k (an anti-diagonal sweep).  Its upper neighbours
-//     (row above, columns c and c-1) are lane k-1's values from the last two steps; they arrive by
-//     DPP wave_shr:1, one v_mov_dpp per state.  Lane 0's upper neighbour is the last row of the
-//     previous stripe, read from an LDS boundary row; the DPP leaves lane 0's "old" operand (that
-//     LDS value) in place, so lane 0 needs no extra VALU work.
-//   * Lane 63 (or, on the last stripe, the lane holding row R) writes its cell back into the same
-//     LDS boundary row, 63 columns behind lane 0's read point, so one buffer serves all stripes.
+//     (row above, columns c and c-1) are lane k-1's values from the last two steps; they arrive as
+//     DPP wave_shr:1 source operands of v_mul / v_fmac (bound_ctrl: lane 0 reads 0), so the
+//     exchange costs no instructions of its own.
+//   * Lane 63 writes its cell back into the same LDS boundary row, 63 columns behind lane 0's read
+//     point, so one buffer serves all stripes.  Rows past R on the last stripe pass the last row
+//     down (M = D = 0, I = M + I of the row above), so lane 63 then writes M(R,c) + I(R,c).
 //   * Cells with c <= 0 or c > H are computed too (they are exact zeros, resp. never read by a real
 //     cell), which keeps the inner loop free of masks.
 //   * The last-row sum, the unscaling and log10 are done in FP64.  FP32 launches use S = 120 and
@@ -91,44 +92,63 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
             pm = static_cast<T>(1.0 - eq);
             px = static_cast<T>(eq / 3.0);
             if (base == 'N') px = pm;
+        } else if (lane > 0) {
+            // Rows past R pass the last row down unchanged: M = D = 0 and I(r,c) = M(r-1,c) + I(r-1,c),
+            // so lane 63 of the last stripe holds M(R,c) + I(R,c) in its I and the writer is always
+            // lane 63 (row 63 (s+1) on the other stripes, the next stripe's relay row).
+            MI = 1;
+            II = 1;
         }
-        // Lane holding row R on the last stripe, else lane 63 (row 63 (s+1), the next relay row).
-        const int writer = (s == n_stripes - 1) ? (R - s * kRows) : kRows;
+        constexpr int writer = kRows;
         T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
         T m2 = 0, s2 = 0;                  // own cell two steps back (column c-2)
         const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
-        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
+        // Lane 63 stores column c = t - 63 at step t; lane 0 loads column c = t.  One base pointer
+        // serves both, the loads 63 elements above the stores.
         T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *ws = L.bs + kLdsPad - writer;
+        // The store and load conditions are two independent loop-invariant lane masks.  Written
+        // on `lane` directly, the compiler merges them into one if / else and re-evaluates both
+        // compares (two VALU) on every step; laundered copies keep them in SGPR pairs.
+        int lane_w = lane, lane_r = lane;
+        asm volatile("" : "+v"(lane_w), "+v"(lane_r));
+        const bool is_writer = lane_w == writer, is_relay = lane_r == 0;
         const int steps = round_up(H + kWave, kUnroll);
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            uint32_t hbs[kUnroll];  // haplotype bytes of the block, read ahead of their use
+            // Haplotype bytes of the block, read ahead of their use as four 8-byte loads at byte
+            // offsets 0..3: step u's byte is then byte 0 of a dword (q[u & 3], dword u >> 2), which
+            // the compare selects with SDWA instead of a separate shift/mask per step.
+            uint64_t q[4];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
+            for (int k = 0; k < 4; ++k) __builtin_memcpy(&q[k], hp + t0 + k, sizeof(uint64_t));
+            T *wmb = wm + t0, *wib = wi + t0, *wsb = ws + t0;
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const int t = t0 + u;
-                const uint32_t hb = hbs[u];
+                const uint32_t hb = static_cast<uint32_t>(q[u & 3] >> (32 * (u >> 2))) & 0xFFu;
                 const bool match = (hb == base) || (kHapHasN && hb == 'N');
                 const T prior = match ? pm : px;
                 T a = from_prev_lane(m2) * MM;
-                a = fma(from_prev_lane(s2), GM, a);
-                T m = a * prior;
                 T i = from_prev_lane(m1) * MI;
-                i = fma(from_prev_lane(i1), II, i);
+                if constexpr (sizeof(T) == sizeof(float)) {
+                    fmac_prev_lane2(a, s2, GM, i, i1, II);
+                } else {
+                    a = fma(from_prev_lane(s2), GM, a);
+                    i = fma(from_prev_lane(i1), II, i);
+                }
+                T m = a * prior;
                 T d = m1 * MD;
                 d = fma(d1, DD, d);
                 T sx = i + d;
                 // The writer stores before lane 0's relay load is issued: the load writes the same
                 // registers, so storing after it would wait for the LDS round trip every step.
-                if (lane == writer) {
-                    wm[t] = m;
-                    wi[t] = i;
-                    ws[t] = sx;
+                if (is_writer) {
+                    wmb[u] = m;
+                    wib[u] = i;
+                    wsb[u] = sx;
                 }
-                if (lane == 0) {
-                    m = um[t];
-                    i = ui[t];
-                    sx = us[t];
+                if (is_relay) {
+                    m = wmb[writer + u];
+                    i = wib[writer + u];
+                    sx = wsb[writer + u];
                 }
                 m2 = m1;
                 s2 = s1;
@@ -234,13 +254,20 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
 // Column capacity classes of the LDS slice (haplotype bytes + three boundary planes).  The class is
 // a template parameter so every LDS access is one base register plus an immediate offset.  A class
 // holds columns -64..H+63+kUnroll plus slack.
-constexpr int kColClasses[] = {384, 768, 1024, 1280, 1664, 2560, 4992};
+constexpr int kColClasses[] = {384, 512, 640, 768, 1024, 1280, 1664, 2560, 4992};
 
 int lds_columns(int max_hap_len) {
     const int need = max_hap_len + 2 * kLdsPad + 32;
     for (int c : kColClasses)
         if (c >= need) return c;
     return 0;
+}
+
+int lds_class_index(int hap_len) {
+    const int need = hap_len + 2 * kLdsPad + 32;
+    for (int k = 0; k < static_cast<int>(sizeof(kColClasses) / sizeof(kColClasses[0])); ++k)
+        if (kColClasses[k] >= need) return k;
+    return -1;
 }
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
@@ -257,6 +284,8 @@ hipError_t with_kernel(bool fp64, int cols, F &&f) {
     case C:                                                                     \
         return fp64 ? f(forward_kernel<double, C>) : f(forward_kernel<float, C>);
         GATK_PAIRHMM_CASE(384)
+        GATK_PAIRHMM_CASE(512)
+        GATK_PAIRHMM_CASE(640)
         GATK_PAIRHMM_CASE(768)
         GATK_PAIRHMM_CASE(1024)
         GATK_PAIRHMM_CASE(1280)

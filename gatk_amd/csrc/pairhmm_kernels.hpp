@@ -50,6 +50,9 @@ struct RescueSink {
 // Bytes of LDS one wavefront needs for haplotypes up to max_hap_len.
 int lds_columns(int max_hap_len);
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64);
+// Index of the LDS column class that holds haplotypes of this length (0 = smallest), -1 if none.
+// Queues split by class let short haplotypes run at the occupancy their smaller LDS slice allows.
+int lds_class_index(int hap_len);
 
 // Dynamic LDS budget per workgroup: the device's limit (queried once), at most 160 KB (gfx950).
 size_t max_lds_per_block();

@@ -91,9 +91,10 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     int32_t st = finalize(s, sl);
     if (st != PAIRHMM_OK) return st;
     std::vector<uint32_t> order;
+    std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    st = validate_and_order(d, ctx->queue_order, order, cells, max_h);
+    st = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices);
     if (st != PAIRHMM_OK) return st;
     const int64_t np = d->n_pairs;
     if (np == 0) return PAIRHMM_OK;
@@ -113,8 +114,8 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
                  o_roff = take(8 * d->n_reads), o_rlen = take(4 * d->n_reads), o_hoff = take(8 * d->n_haps),
                  o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np);
     const size_t in_bytes = off;
-    const size_t o_out = take(8 * np), o_cnt = take(4 * 16), o_res = take(4 * np);
-    if (sl.hin.reserve(in_bytes) != hipSuccess || sl.hout.reserve(8 * np + 64) != hipSuccess ||
+    const size_t o_out = take(8 * np), o_cnt = take(4 * kCounterSlots), o_res = take(4 * np);
+    if (sl.hin.reserve(in_bytes) != hipSuccess || sl.hout.reserve(8 * np + 4 * kCounterSlots) != hipSuccess ||
         sl.dmem.reserve(off) != hipSuccess)
         return fail(PAIRHMM_ERR_OOM, "stream slot allocation failed");
     uint8_t *h = static_cast<uint8_t *>(sl.hin.ptr);
@@ -153,21 +154,16 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     uint32_t *cnt = reinterpret_cast<uint32_t *>(m + o_cnt);
     uint32_t *res = reinterpret_cast<uint32_t *>(m + o_res);
     DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
-    const int cus = std::max(ctx->num_cus, 1);
-    const int grid32 = std::max(1, sh32.blocks_per_cu * cus * ctx->grid_percent / 100);
-    const int grid64 = std::max(1, sh64.blocks_per_cu * cus * ctx->grid_percent / 100);
     hipStream_t cs = ctx->stream;
     HIP_TRY(hipStreamWaitEvent(cs, sl.h2d_done, 0), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(launch_zero_counters(cnt, 16, cs), PAIRHMM_ERR_KERNEL);
-    WorkSource q{ord, static_cast<uint32_t>(np), nullptr, cnt + 0};
-    RescueSink rs{res, cnt + 1, kFp32RescueThreshold};
-    HIP_TRY(launch_forward(false, v, tab, q, rs, max_h, sh32, grid32, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_zero_counters(cnt, kCounterSlots, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs), PAIRHMM_ERR_KERNEL);
     WorkSource q2{res, 0, cnt + 1, cnt + 2};
     RescueSink none{nullptr, nullptr, 0.0};
-    HIP_TRY(launch_forward(true, v, tab, q2, none, max_h, sh64, grid64, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_forward(true, v, tab, q2, none, max_h, sh64, persistent_grid(ctx, sh64, np), cs), PAIRHMM_ERR_KERNEL);
     double *ho = static_cast<double *>(sl.hout.ptr);
     HIP_TRY(hipMemcpyAsync(ho, v.out, 8 * np, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(hipMemcpyAsync(ho + np, cnt, 64, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipMemcpyAsync(ho + np, cnt, 4 * kCounterSlots, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipEventRecord(sl.done, cs), PAIRHMM_ERR_KERNEL);
     sl.busy = true;
     sl.user_out = out_host;
