@@ -418,30 +418,27 @@ size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_s
 
 namespace {
 
+constexpr int kMaxGradCols = 2560;  // longer haplotypes exceed the adjoint's LDS and scratch plan
+
 template <typename F>
 hipError_t with_grad_kernel(bool fp64, int cols, F &&f) {
-    switch (cols) {
-#define GATK_PAIRHMM_GRAD_CASE(C)                                                                 \
-    case C:                                                                                       \
-        return fp64 ? f(grad_kernel<double, C, kGradTile>) : f(grad_kernel<float, C, kGradTile>);
-        GATK_PAIRHMM_GRAD_CASE(384)
-        GATK_PAIRHMM_GRAD_CASE(768)
-        GATK_PAIRHMM_GRAD_CASE(1024)
-        GATK_PAIRHMM_GRAD_CASE(1280)
-        GATK_PAIRHMM_GRAD_CASE(1664)
-        GATK_PAIRHMM_GRAD_CASE(2560)
-#undef GATK_PAIRHMM_GRAD_CASE
-        default:
+    return dispatch_col_class(cols, [&](auto C) -> hipError_t {
+        constexpr int kC = decltype(C)::value;
+        if constexpr (kC > kMaxGradCols) {
             return hipErrorInvalidValue;
-    }
+        } else {
+            return fp64 ? f(grad_kernel<double, kC, kGradTile>) : f(grad_kernel<float, kC, kGradTile>);
+        }
+    });
 }
+
 
 }  // namespace
 
 LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
     LaunchShape sh;
     const int cols = lds_columns(max_hap_len);
-    if (cols == 0 || cols > 2560) return sh;
+    if (cols == 0 || cols > kMaxGradCols) return sh;
     const size_t per_wave = grad_lds_bytes_per_wave(max_hap_len, fp64);
     const size_t budget = max_lds_per_block();
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;

@@ -88,6 +88,33 @@ __device__ __forceinline__ void fmac_prev_lane2(float &a, float src_a, float mul
         : "v"(src_a), "v"(mul_a), "v"(src_b), "v"(mul_b));
 }
 
+// Lane 0's value, wave-uniform (v_readfirstlane).
+__device__ __forceinline__ float rfl(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double rfl(double x) {
+    const uint64_t v = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<int>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Lanes 1..63: a = a0(lane k-1) * ma0 + a1(lane k-1) * ma1 and b = b0(lane k-1) * mb0 + b1(lane k-1) * mb1.
+// Lane 0 keeps the a and b it brings in: a DPP wave_shr:1 without bound_ctrl leaves a lane whose
+// source lies outside the wave unwritten.  Four VALU instructions, the lane exchange included; the
+// leading s_nop 4 provides the DPP wait states the compiler does not track across inline assembly
+// (2 after a VALU write of a DPP source, 5 after an EXEC write).
+__device__ __forceinline__ void dpp_upper2(float &a, float a0, float ma0, float a1, float ma1, float &b, float b0,
+                                           float mb0, float b1, float mb1) {
+    asm("s_nop 4\n\t"
+        "v_mul_f32_dpp %0, %2, %3 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %6, %7 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %4, %5 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %8, %9 wave_shr:1 row_mask:0xf bank_mask:0xf"
+        : "+&v"(a), "+&v"(b)
+        : "v"(a0), "v"(ma0), "v"(a1), "v"(ma1), "v"(b0), "v"(mb0), "v"(b1), "v"(mb1));
+}
+
 constexpr int kRows = kWave - 1;  // real rows per stripe: lane 0 relays the row above
 
 }  // namespace dev

@@ -40,11 +40,11 @@ struct WaveLds {
     T *bm, *bi, *bs;   // [cols]: boundary row M, I and S = I + D, column c at kLdsPad + c
 };
 
-// Boundary planes are kCols + 1 elements apart: an odd stride keeps the compiler on single
-// ds_read_b32 / ds_write_b32 with one shared base register and immediate offsets (a multiple of 64
-// dwords would turn them into ds_read2st64 pairs that need an extra address add per step).
+// Boundary planes are kCols + 4 elements apart: 16-byte aligned (lane 0's A / B reads are
+// vector loads) and not a multiple of 64 dwords, so lane 63's three stores stay single
+// ds_write_b32 off one base register rather than ds_write2st64 pairs with their own address adds.
 template <int kCols>
-constexpr int plane_stride() { return kCols + 1; }
+constexpr int plane_stride() { return kCols + 4; }
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() { return static_cast<size_t>(kCols) + 3 * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T); }
@@ -62,23 +62,29 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
 
 // One pass over a pair, stripe by stripe.  Returns the scaled last-row sum (FP64, every lane).
 //
-// Stripe s covers rows 63 s + 1 .. 63 s + 63 on lanes 1..63; lane 0 relays row 63 s, read from the
-// LDS boundary row, so every real row gets its upper neighbours from lane k-1 by DPP.  Per step
-// each lane keeps its own M, I, D from the previous step and M, S = I + D from two steps back:
+// Stripe s covers rows 64 s + 1 .. 64 s + 64, one per lane.  Per step each lane keeps its own M, I,
+// D from the previous step and M, S = I + D from two steps back:
 //   M(r,c) = prior * (M(r-1,c-1) MM + S(r-1,c-1) GM)     [lane k-1, two steps back]
 //   I(r,c) = M(r-1,c) MI + I(r-1,c) II                    [lane k-1, one step back]
 //   D(r,c) = M(r,c-1) MD + D(r,c-1) DD                    [own lane, one step back]
+// Lane 0's row above is the last row of the previous stripe, which lane 63 left in the LDS
+// boundary planes.  Before the stripe a side pass turns those planes into lane 0's two sums,
+//   A(c) = M(r-1,c-1) MM + S(r-1,c-1) GM   and   B(c) = M(r-1,c) MI + I(r-1,c) II,
+// in place (A over S, B over I).  Lane 0 reads them ahead with wave-uniform loads and brings them
+// into the DPP products, which leave lane 0 unwritten, so no step waits on LDS.
 template <typename T, int kCols, bool kHapHasN>
 __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
                                const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
                                const DeviceTables &tab) {
-    const int n_stripes = (R + kRows - 1) / kRows;
+    constexpr int kStripe = kWave;
+    const int n_stripes = (R + kStripe - 1) / kStripe;
+    const int steps = round_up(H + kWave, kUnroll);
     for (int s = 0; s < n_stripes; ++s) {
-        const int p = s * kRows + lane - 1;  // read position of this lane's row
+        const int p = s * kStripe + lane;  // read position of this lane's row
         T MM = 0, GM = 0, MI = 0, II = 0, MD = 0, DD = 0, pm = 0, px = 0;
         uint32_t base = 0;
-        if (lane > 0 && p < R) {
+        if (p < R) {
             base = rb[p];
             const int q = rq[p], qi = ri[p], qd = rd[p], qc = rg[p];
             const double eq = tab.qual_err[q], ei = tab.qual_err[qi], ed = tab.qual_err[qd], ec = tab.qual_err[qc];
@@ -92,63 +98,74 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
             pm = static_cast<T>(1.0 - eq);
             px = static_cast<T>(eq / 3.0);
             if (base == 'N') px = pm;
-        } else if (lane > 0) {
+        } else {
             // Rows past R pass the last row down unchanged: M = D = 0 and I(r,c) = M(r-1,c) + I(r-1,c),
-            // so lane 63 of the last stripe holds M(R,c) + I(R,c) in its I and the writer is always
-            // lane 63 (row 63 (s+1) on the other stripes, the next stripe's relay row).
+            // so lane 63 of the last stripe holds M(R,c) + I(R,c) in its I.
             MI = 1;
             II = 1;
         }
-        constexpr int writer = kRows;
+        // Side pass: lane 0's sums A, B from the boundary row, highest columns first so that every
+        // read of index j-1 precedes its overwrite.
+        {
+            const T MM0 = rfl(MM), GM0 = rfl(GM), MI0 = rfl(MI), II0 = rfl(II);
+            __builtin_amdgcn_wave_barrier();
+            for (int j0 = kCols - kWave; j0 >= 0; j0 -= kWave) {
+                const int j = j0 + lane;
+                const T mj = L.bm[j], ij = L.bi[j];
+                const T mp = j > 0 ? L.bm[j - 1] : T(0), sp = j > 0 ? L.bs[j - 1] : T(0);
+                __builtin_amdgcn_wave_barrier();
+                L.bs[j] = fma(sp, GM0, mp * MM0);
+                L.bi[j] = fma(ij, II0, mj * MI0);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
         T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
         T m2 = 0, s2 = 0;                  // own cell two steps back (column c-2)
-        const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
-        // Lane 63 stores column c = t - 63 at step t; lane 0 loads column c = t.  One base pointer
-        // serves both, the loads 63 elements above the stores.
-        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *ws = L.bs + kLdsPad - writer;
-        // The store and load conditions are two independent loop-invariant lane masks.  Written
-        // on `lane` directly, the compiler merges them into one if / else and re-evaluates both
-        // compares (two VALU) on every step; laundered copies keep them in SGPR pairs.
+        // Byte loads of the haplotype, eight per block.  volatile keeps them from being merged into
+        // unaligned 8-byte loads; the explicit LDS address space keeps them ds_read_u8.
+        using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
+        lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
+        // Lane 63 stores column c = t - 63 at step t; lane 0 reads A, B of column c = t.  One base
+        // pointer serves both, the reads 63 elements above the stores.
+        constexpr int kLag = kWave - 1;
+        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *ws = L.bs + kLdsPad - kLag;
         int lane_w = lane, lane_r = lane;
         asm volatile("" : "+v"(lane_w), "+v"(lane_r));
-        const bool is_writer = lane_w == writer, is_relay = lane_r == 0;
-        const int steps = round_up(H + kWave, kUnroll);
+        const bool is_writer = lane_w == kLag;
+        const bool is_first = lane_r == 0;
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            // Haplotype bytes of the block, read ahead of their use as four 8-byte loads at byte
-            // offsets 0..3: step u's byte is then byte 0 of a dword (q[u & 3], dword u >> 2), which
-            // the compare selects with SDWA instead of a separate shift/mask per step.
-            uint64_t q[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) __builtin_memcpy(&q[k], hp + t0 + k, sizeof(uint64_t));
             T *wmb = wm + t0, *wib = wi + t0, *wsb = ws + t0;
+            T av[kUnroll], bv[kUnroll];
+            __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
+            __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
+            uint32_t hbs[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t hb = static_cast<uint32_t>(q[u & 3] >> (32 * (u >> 2))) & 0xFFu;
+                const uint32_t hb = hbs[u];
                 const bool match = (hb == base) || (kHapHasN && hb == 'N');
                 const T prior = match ? pm : px;
-                T a = from_prev_lane(m2) * MM;
-                T i = from_prev_lane(m1) * MI;
+                T a, i;
                 if constexpr (sizeof(T) == sizeof(float)) {
-                    fmac_prev_lane2(a, s2, GM, i, i1, II);
+                    a = av[u];
+                    i = bv[u];
+                    dpp_upper2(a, m2, MM, s2, GM, i, m1, MI, i1, II);
                 } else {
-                    a = fma(from_prev_lane(s2), GM, a);
-                    i = fma(from_prev_lane(i1), II, i);
+                    a = fma(from_prev_lane(s2), GM, from_prev_lane(m2) * MM);
+                    i = fma(from_prev_lane(i1), II, from_prev_lane(m1) * MI);
+                    a = is_first ? av[u] : a;
+                    i = is_first ? bv[u] : i;
                 }
-                T m = a * prior;
+                const T m = a * prior;
                 T d = m1 * MD;
                 d = fma(d1, DD, d);
-                T sx = i + d;
-                // The writer stores before lane 0's relay load is issued: the load writes the same
-                // registers, so storing after it would wait for the LDS round trip every step.
+                const T sx = i + d;
                 if (is_writer) {
                     wmb[u] = m;
                     wib[u] = i;
                     wsb[u] = sx;
-                }
-                if (is_relay) {
-                    m = wmb[writer + u];
-                    i = wib[writer + u];
-                    sx = wsb[writer + u];
                 }
                 m2 = m1;
                 s2 = s1;
@@ -158,6 +175,7 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
                 s1 = sx;
             }
         }
+        __builtin_amdgcn_wave_barrier();
     }
     double acc = 0.0;
     for (int c = 1 + lane; c <= H; c += kWave)
@@ -195,10 +213,11 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
         L.hap[j] = v;
     }
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
+    // Row 0: M = I = 0 and D = S = 2^S / H from column 0 on; columns left of 0 are zero.
     for (int j = lane; j < kCols; j += kWave) {
         L.bm[j] = 0;
         L.bi[j] = 0;
-        L.bs[j] = d0;
+        L.bs[j] = j >= kLdsPad ? d0 : T(0);
     }
     double sum;
     if (__any(has_n))
@@ -251,11 +270,6 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
 
 }  // namespace
 
-// Column capacity classes of the LDS slice (haplotype bytes + three boundary planes).  The class is
-// a template parameter so every LDS access is one base register plus an immediate offset.  A class
-// holds columns -64..H+63+kUnroll plus slack.
-constexpr int kColClasses[] = {384, 512, 640, 768, 1024, 1280, 1664, 2560, 4992};
-
 int lds_columns(int max_hap_len) {
     const int need = max_hap_len + 2 * kLdsPad + 32;
     for (int c : kColClasses)
@@ -265,7 +279,7 @@ int lds_columns(int max_hap_len) {
 
 int lds_class_index(int hap_len) {
     const int need = hap_len + 2 * kLdsPad + 32;
-    for (int k = 0; k < static_cast<int>(sizeof(kColClasses) / sizeof(kColClasses[0])); ++k)
+    for (int k = 0; k < kNumColClasses; ++k)
         if (kColClasses[k] >= need) return k;
     return -1;
 }
@@ -279,24 +293,11 @@ namespace {
 
 template <typename F>
 hipError_t with_kernel(bool fp64, int cols, F &&f) {
-    switch (cols) {
-#define GATK_PAIRHMM_CASE(C)                                                    \
-    case C:                                                                     \
-        return fp64 ? f(forward_kernel<double, C>) : f(forward_kernel<float, C>);
-        GATK_PAIRHMM_CASE(384)
-        GATK_PAIRHMM_CASE(512)
-        GATK_PAIRHMM_CASE(640)
-        GATK_PAIRHMM_CASE(768)
-        GATK_PAIRHMM_CASE(1024)
-        GATK_PAIRHMM_CASE(1280)
-        GATK_PAIRHMM_CASE(1664)
-        GATK_PAIRHMM_CASE(2560)
-        GATK_PAIRHMM_CASE(4992)
-#undef GATK_PAIRHMM_CASE
-        default:
-            return hipErrorInvalidValue;
-    }
+    return dispatch_col_class(cols, [&](auto C) {
+        return fp64 ? f(forward_kernel<double, decltype(C)::value>) : f(forward_kernel<float, decltype(C)::value>);
+    });
 }
+
 
 }  // namespace
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstdint>
 
 namespace gatk_pairhmm {
@@ -46,6 +48,24 @@ struct RescueSink {
     uint32_t *count;
     double threshold;
 };
+
+// Column capacity classes of a wave's LDS slice (haplotype bytes + boundary planes).  The class is
+// a template parameter of every kernel so each LDS access is one base register plus an immediate
+// offset.  A class holds columns -64..H+63+8 plus slack: H + 160 <= class.
+inline constexpr int kColClasses[] = {384, 512, 640, 768, 1024, 1280, 1664, 2560, 4992};
+inline constexpr int kNumColClasses = static_cast<int>(sizeof(kColClasses) / sizeof(kColClasses[0]));
+
+// Calls f(std::integral_constant<int, C>{}) for the class C == cols; hipErrorInvalidValue if cols
+// is not a class.  Every kernel family instantiates through this, so all share one class list.
+template <int I = 0, typename F>
+hipError_t dispatch_col_class(int cols, F &&f) {
+    if constexpr (I == kNumColClasses) {
+        return hipErrorInvalidValue;
+    } else {
+        if (cols == kColClasses[I]) return f(std::integral_constant<int, kColClasses[I]>{});
+        return dispatch_col_class<I + 1>(cols, static_cast<F &&>(f));
+    }
+}
 
 // Bytes of LDS one wavefront needs for haplotypes up to max_hap_len.
 int lds_columns(int max_hap_len);

@@ -80,6 +80,8 @@ extern "C" int32_t pairhmm_stream_create(pairhmm_context *ctx, pairhmm_stream **
 extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_desc *d, double *out_host) {
     if (!s || !s->ctx) return fail(PAIRHMM_ERR_STATE, "stream is null");
     if (!d || !out_host) return fail(PAIRHMM_ERR_BAD_ARG, "null argument");
+    if (d->n_pairs < 0) return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
+    if (d->n_pairs == 0) return PAIRHMM_OK;  // nothing to compute; no slot is used
     pairhmm_context *ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
     if (!s->started) {
