@@ -170,9 +170,9 @@ def main() -> int:
     ap.add_argument("--bwd-regions", type=int, default=2_000,
                     help="regions of the forward+backward leg (0 skips it)")
     ap.add_argument("--bwd-steps", type=int, default=2)
-    ap.add_argument("--fit-regions", type=int, default=2_000,
+    ap.add_argument("--fit-regions", type=int, default=1_000,
                     help="regions of the STR penalty fit leg (100 reads each; 0 skips it)")
-    ap.add_argument("--fit-iters", type=int, default=60)
+    ap.add_argument("--fit-iters", type=int, default=200)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-schedule autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 def test_fit_recovers_str_penalties():
     spec = strfit.FitSpec(n_regions=500, reads_per_region=80)
     print(f"[strfit] spec={spec} seed=77", flush=True)
-    wl, res, rep = strfit.run_gpu_fit(spec, seed=77, max_iter=80,
+    wl, res, rep = strfit.run_gpu_fit(spec, seed=77, max_iter=200,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"[strfit] {res.evaluations} evaluations in {res.seconds:.1f} s; report {rep.as_dict()}", flush=True)
     assert np.isfinite(res.loss)
