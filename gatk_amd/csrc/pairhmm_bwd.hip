@@ -100,7 +100,7 @@ struct GradLds {
 };
 
 template <int kCols>
-constexpr int grad_stride() { return kCols + 1; }
+constexpr int grad_stride() { return lds_plane_stride(kCols); }
 
 template <typename T, int kCols>
 constexpr size_t grad_lds_per_wave() {
@@ -119,34 +119,96 @@ __device__ __forceinline__ GradLds<T, kCols> grad_carve(uint8_t *base) {
     return w;
 }
 
+// Lane 0's sums of the row above (A = M(c-1) MM + S(c-1) GM, B = M(c) MI + I(c) II) from the boundary
+// planes, in place: A over S, B over I, highest columns first so every read of index j-1 precedes
+// its overwrite.  The row above is in bm / bi / bs; MM0..II0 are lane 0's row parameters.
+template <typename T, int kCols>
+__device__ __forceinline__ void side_pass(const GradLds<T, kCols> &L, int lane, T MM0, T GM0, T MI0, T II0) {
+    __builtin_amdgcn_wave_barrier();
+    for (int j0 = kCols - kWave; j0 >= 0; j0 -= kWave) {
+        const int j = j0 + lane;
+        const T mj = L.bm[j], ij = L.bi[j];
+        const T mp = j > 0 ? L.bm[j - 1] : T(0), sp = j > 0 ? L.bs[j - 1] : T(0);
+        __builtin_amdgcn_wave_barrier();
+        L.bs[j] = fma(sp, GM0, mp * MM0);
+        L.bi[j] = fma(ij, II0, mj * MI0);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One forward step of every lane (rows of a 64-row stripe, lane k at column t - k).  Lane 0 takes
+// its sums of the row above from a0 / b0; the other lanes take lane k-1's values by DPP.  Returns
+// the pre-prior sum a (for the base-error adjoint); updates the rolling state.
+template <typename T, bool kHapHasN>
+__device__ __forceinline__ T fwd_step(const Row<T> &w, T px, uint32_t hb, T a0, T b0, bool is_first, T &m1, T &i1,
+                                      T &d1, T &s1, T &m2, T &s2, T &m_out, T &i_out, T &d_out, T &s_out, T &p_out) {
+    const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
+    const T prior = match ? w.pm : px;
+    T a, i;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        a = a0;
+        i = b0;
+        dpp_upper2(a, m2, w.MM, s2, w.GM, i, m1, w.MI, i1, w.II);
+    } else {
+        a = fma(from_prev_lane(s2), w.GM, from_prev_lane(m2) * w.MM);
+        i = fma(from_prev_lane(i1), w.II, from_prev_lane(m1) * w.MI);
+        a = is_first ? a0 : a;
+        i = is_first ? b0 : i;
+    }
+    const T m = a * prior;
+    T d = m1 * w.MD;
+    d = fma(d1, w.DD, d);
+    const T sx = i + d;
+    m_out = m;
+    i_out = i;
+    d_out = d;
+    s_out = sx;
+    p_out = prior;
+    m2 = m1;
+    s2 = s1;
+    m1 = m;
+    i1 = i;
+    d1 = d;
+    s1 = sx;
+    return a;
+}
+
 // One pair: forward with checkpoints, then the adjoint.  Returns nothing; writes the outputs.
+//
+// Stripes of 64 rows, lane k = read position 64 s + k; rows past R pass the row above down
+// (MI = II = 1), in the forward and in the adjoint.  Scratch per stripe: the state registers every
+// kTile steps (checkpoints) and lane 0's A / B planes of the stripe.
 template <typename T, int kCols, int kTile, bool kHapHasN>
 __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__restrict__ scratch, int lane,
                           uint32_t pair, int H, int R, int64_t ro, double weight, const RescueSink &rs) {
-    const int n_stripes = (R + kRows - 1) / kRows;
+    constexpr int kStripe = kWave;
+    constexpr int kLag = kWave - 1;
+    const int n_stripes = (R + kStripe - 1) / kStripe;
     const int steps = round_up(H + kWave, kTile);
     const int n_tiles = steps / kTile;
     T *ck = scratch;                                                        // [stripe][tile][6][64]
-    T *relay = scratch + static_cast<size_t>(g.max_stripes) * g.max_tiles * 6 * kWave;  // [stripe][3][kCols]
+    T *abp = scratch + static_cast<size_t>(g.max_stripes) * g.max_tiles * 6 * kWave;  // [stripe][2][kCols]
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
-    const uint8_t *hp = L.hap + kLdsPad - 1 - lane;
+    using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
+    lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
+    int lane_w = lane, lane_r = lane, lane_x = lane;
+    asm volatile("" : "+v"(lane_w), "+v"(lane_r), "+v"(lane_x));
+    const bool is_writer = lane_w == kLag, is_first = lane_r == 0;
 
     // ---- 1. forward, top down --------------------------------------------------------------
     for (int s = 0; s < n_stripes; ++s) {
-        if (s > 0) {
-            T *dst = relay + static_cast<size_t>(s) * 3 * kCols;
+        const Row<T> w = load_row<T>(g, ro, R, s * kStripe + lane, true);
+        const T px = w.n_read ? w.pm : w.px;
+        side_pass<T, kCols>(L, lane, rfl(w.MM), rfl(w.GM), rfl(w.MI), rfl(w.II));
+        {
+            T *dst = abp + static_cast<size_t>(s) * 2 * kCols;
             for (int j = lane; j < kCols; j += kWave) {
-                dst[j] = L.bm[j];
+                dst[j] = L.bs[j];
                 dst[kCols + j] = L.bi[j];
-                dst[2 * kCols + j] = L.bs[j];
             }
         }
-        const Row<T> w = load_row<T>(g, ro, R, s * kRows + lane - 1, lane > 0);
-        const T px = w.n_read ? w.pm : w.px;
-        const int writer = (s == n_stripes - 1) ? (R - s * kRows) : kRows;
         T m1 = 0, i1 = 0, d1 = 0, s1 = 0, m2 = 0, s2 = 0;
-        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
-        T *wm = L.bm + kLdsPad - writer, *wi = L.bi + kLdsPad - writer, *wsp = L.bs + kLdsPad - writer;
+        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *wsp = L.bs + kLdsPad - kLag;
         T *ckp = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
         for (int t0 = 0; t0 < steps; t0 += kTile) {
             ckp[0 * kWave] = m1;
@@ -156,38 +218,26 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             ckp[4 * kWave] = m2;
             ckp[5 * kWave] = s2;
             ckp += 6 * kWave;
+            T av[kTile], bv[kTile];
+            __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
+            __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
+            uint32_t hbs[kTile];
+#pragma unroll
+            for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
+            __builtin_amdgcn_wave_barrier();
+            T *wmb = wm + t0, *wib = wi + t0, *wsb = wsp + t0;
 #pragma unroll
             for (int u = 0; u < kTile; ++u) {
-                const int t = t0 + u;
-                const uint32_t hb = hp[t];
-                const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
-                const T prior = match ? w.pm : px;
-                T a = from_prev_lane(m2) * w.MM;
-                a = fma(from_prev_lane(s2), w.GM, a);
-                T m = a * prior;
-                T i = from_prev_lane(m1) * w.MI;
-                i = fma(from_prev_lane(i1), w.II, i);
-                T d = m1 * w.MD;
-                d = fma(d1, w.DD, d);
-                T sx = i + d;
-                if (lane == writer) {
-                    wm[t] = m;
-                    wi[t] = i;
-                    wsp[t] = sx;
+                T m, i, d, sx, pr;
+                fwd_step<T, kHapHasN>(w, px, hbs[u], av[u], bv[u], is_first, m1, i1, d1, s1, m2, s2, m, i, d, sx, pr);
+                if (is_writer) {
+                    wmb[u] = m;
+                    wib[u] = i;
+                    wsb[u] = sx;
                 }
-                if (lane == 0) {
-                    m = um[t];
-                    i = ui[t];
-                    sx = us[t];
-                }
-                m2 = m1;
-                s2 = s1;
-                m1 = m;
-                i1 = i;
-                d1 = d;
-                s1 = sx;
             }
         }
+        __builtin_amdgcn_wave_barrier();
     }
     double acc = 0.0;
     for (int c = 1 + lane; c <= H; c += kWave)
@@ -212,6 +262,7 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
     if (!(l_scaled > 0.0)) return;  // FP64 underflow: log10 L = -inf, no gradient
     const T seed = static_cast<T>(__builtin_ldexp(1.0, sigma));
     const double norm = weight / (l_scaled * __builtin_ldexp(1.0, sigma) * 2.30258509299404568402);
+    const double dq = -0.23025850929940456840;  // d e(q)/dq = dq * e(q)
 
     // Adjoint boundary below the last stripe: the seed row (gI = seed on columns 1..H, X = 0).
     for (int j = lane; j < kCols; j += kWave) {
@@ -222,75 +273,54 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 
     // ---- 2. adjoint, bottom up -------------------------------------------------------------------
     for (int s = n_stripes - 1; s >= 0; --s) {
-        const int p = s * kRows + lane - 1;
-        const Row<T> w = load_row<T>(g, ro, R, p, lane > 0);
+        const int p = s * kStripe + lane;
+        const Row<T> w = load_row<T>(g, ro, R, p, true);
         const T px = w.n_read ? w.pm : w.px;
         // Next row's MM, GM, MI, II: lane k+1's own; lane 63's is the first row of the stripe below
-        // (or the seed row after the last stripe).
-        Row<T> below{T(0), T(0), T(1), T(1), T(0), T(0), T(0), T(0), 0u, false};
-        if (s < n_stripes - 1) below = load_row<T>(g, ro, R, (s + 1) * kRows, true);
+        // (a pass-through row below the last stripe).
+        const int p_below = (s + 1) * kStripe;
+        const Row<T> below = load_row<T>(g, ro, R, p_below, true);
         const T MMn = wave_shl1(below.MM, w.MM), GMn = wave_shl1(below.GM, w.GM);
         const T MIn = wave_shl1(below.MI, w.MI), IIn = wave_shl1(below.II, w.II);
-        // Relay row of this stripe into LDS (row 0 for the first stripe).
+        // Lane 0's A / B planes of this stripe, for the recompute.
         {
-            const T *src = relay + static_cast<size_t>(s) * 3 * kCols;
+            const T *src = abp + static_cast<size_t>(s) * 2 * kCols;
+            __builtin_amdgcn_wave_barrier();
             for (int j = lane; j < kCols; j += kWave) {
-                if (s == 0) {
-                    L.bm[j] = 0;
-                    L.bi[j] = 0;
-                    L.bs[j] = d0;
-                } else {
-                    L.bm[j] = src[j];
-                    L.bi[j] = src[kCols + j];
-                    L.bs[j] = src[2 * kCols + j];
-                }
+                L.bs[j] = src[j];
+                L.bi[j] = src[kCols + j];
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        const T *um = L.bm + kLdsPad, *ui = L.bi + kLdsPad, *us = L.bs + kLdsPad;
-        const T *bxr = L.bx + kLdsPad - 62, *bgr = L.bg + kLdsPad - 63;
-        T *bxw = L.bx + kLdsPad - 1, *bgw = L.bg + kLdsPad - 1;
+        // Lane 63 reads X(r+1, c+1) and gI(r+1, c) of the row below; lane 0 writes X and gI of this
+        // stripe's first row for the stripe above (column c = t at step t).
+        const T *bxr = L.bx + kLdsPad - (kLag - 1), *bgr = L.bg + kLdsPad - kLag;
+        T *bxw = L.bx + kLdsPad, *bgw = L.bg + kLdsPad;
         T X1 = 0, X2 = 0, gI1 = 0, gD1 = 0;
         T aMMn = 0, aGMn = 0, aMIn = 0, aIIn = 0, aMD = 0, aDD = 0, aE = 0;
         const T *ckbase = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
+        const bool is_x = lane_x == 0;
         for (int tile = n_tiles - 1; tile >= 0; --tile) {
             const int t0 = tile * kTile;
             const T *ckp = ckbase + static_cast<size_t>(tile) * 6 * kWave;
             T m1 = ckp[0 * kWave], i1 = ckp[1 * kWave], d1 = ckp[2 * kWave], s1 = ckp[3 * kWave];
             T m2 = ckp[4 * kWave], s2 = ckp[5 * kWave];
             const T m_before = m1, d_before = d1;
+            T av[kTile], bv[kTile];
+            __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
+            __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
+            uint32_t hbs[kTile];
+#pragma unroll
+            for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
+            __builtin_amdgcn_wave_barrier();
             T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stP[kTile], stAW[kTile];
 #pragma unroll
             for (int u = 0; u < kTile; ++u) {
-                const int t = t0 + u;
-                const uint32_t hb = hp[t];
-                const bool hit = (hb == w.base) || (kHapHasN && hb == 'N');
-                const T prior = hit ? w.pm : px;
+                const T a = fwd_step<T, kHapHasN>(w, px, hbs[u], av[u], bv[u], is_first, m1, i1, d1, s1, m2, s2,
+                                                  stM[u], stI[u], stD[u], stS[u], stP[u]);
+                const bool hit = (hbs[u] == w.base) || (kHapHasN && hbs[u] == 'N');
                 const T wgt = (hit || w.n_read) ? T(-1) : T(1) / T(3);  // d prior / d e(qb)
-                T a = from_prev_lane(m2) * w.MM;
-                a = fma(from_prev_lane(s2), w.GM, a);
-                T m = a * prior;
-                T i = from_prev_lane(m1) * w.MI;
-                i = fma(from_prev_lane(i1), w.II, i);
-                T d = m1 * w.MD;
-                d = fma(d1, w.DD, d);
-                T sx = i + d;
-                if (lane == 0) {
-                    m = um[t];
-                    i = ui[t];
-                    sx = us[t];
-                }
-                stM[u] = m;
-                stI[u] = i;
-                stD[u] = d;
-                stS[u] = sx;
-                stP[u] = prior;
                 stAW[u] = a * wgt;
-                m2 = m1;
-                s2 = s1;
-                m1 = m;
-                i1 = i;
-                d1 = d;
-                s1 = sx;
             }
 #pragma unroll
             for (int u = kTile - 1; u >= 0; --u) {
@@ -311,7 +341,7 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 aMD = fma(gD, Mp, aMD);
                 aDD = fma(gD, Dp, aDD);
                 aE = fma(gM, stAW[u], aE);
-                if (lane == 1) {
+                if (is_x) {
                     bxw[t] = X;
                     bgw[t] = gI;
                 }
@@ -321,15 +351,17 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 gD1 = gD;
             }
         }
-        // The next-row sums belong to the lane below: shift them down one lane.
+        __builtin_amdgcn_wave_barrier();
+        // The next-row sums belong to the lane below: shift them down one lane.  Lane 63's belong to
+        // the first row of the stripe below; lane 0's own next-row terms come from the stripe above
+        // (or, for row 1, from row 0: S(0, c) = d0, so d L / d GM_1 = d0 sum_c X(1, c)).
         const double gMM = static_cast<double>(wave_shr1(T(0), aMMn));
         const double gGM = static_cast<double>(wave_shr1(T(0), aGMn));
         const double gMI = static_cast<double>(wave_shr1(T(0), aMIn));
         const double gII = static_cast<double>(wave_shr1(T(0), aIIn));
-        if (lane > 0 && p < R) {
+        if (p < R) {
             const double ei = static_cast<double>(w.MI), ed = static_cast<double>(w.MD), ec = static_cast<double>(w.II);
             const double eb = 1.0 - static_cast<double>(w.pm);
-            const double dq = -0.23025850929940456840;  // d e(q)/dq = dq * e(q)
             const double d_ei = (-gMM + gMI) * norm, d_ed = (-gMM + static_cast<double>(aMD)) * norm;
             const double d_ec = (-gGM + gII + static_cast<double>(aDD)) * norm;
             const double d_eb = static_cast<double>(aE) * norm;
@@ -338,6 +370,23 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             atomicAdd(&g.g_qc[ro + p], d_ec * dq * ec);
             atomicAdd(&g.g_qb[ro + p], d_eb * dq * eb);
             if (g.g_eb) atomicAdd(&g.g_eb[ro + p], d_eb);
+        }
+        if (lane == kLag && p_below < R) {
+            const double ei = static_cast<double>(below.MI), ed = static_cast<double>(below.MD);
+            const double ec = static_cast<double>(below.II);
+            const double mmn = static_cast<double>(aMMn);
+            atomicAdd(&g.g_qi[ro + p_below], (-mmn + static_cast<double>(aMIn)) * norm * dq * ei);
+            atomicAdd(&g.g_qd[ro + p_below], -mmn * norm * dq * ed);
+            atomicAdd(&g.g_qc[ro + p_below], (-static_cast<double>(aGMn) + static_cast<double>(aIIn)) * norm * dq * ec);
+        }
+        if (s == 0) {
+            double sx = 0.0;
+            for (int c = 1 + lane; c <= H; c += kWave) sx += static_cast<double>(L.bx[kLdsPad + c]);
+            sx = wave_sum(sx);
+            if (lane == 0) {
+                const double ec = static_cast<double>(w.II);
+                atomicAdd(&g.g_qc[ro], -static_cast<double>(d0) * sx * norm * dq * ec);
+            }
         }
     }
 }
@@ -352,7 +401,7 @@ __device__ void grad_process(const GradBatch &g, const RescueSink &rs, uint8_t *
     GradLds<T, kCols> L = grad_carve<T, kCols>(lds);
     // Guards: the scratch and LDS were sized from the caller's max_read_len / max_hap_len; a pair
     // outside those bounds (or with an empty haplotype) gets NaN instead of touching memory.
-    const int n_stripes = (R + kRows - 1) / kRows;
+    const int n_stripes = (R + kWave - 1) / kWave;
     const int n_tiles = round_up(H + kWave, kTile) / kTile;
     if (H <= 0 || H + 2 * kLdsPad + 32 > kCols || n_stripes > g.max_stripes || n_tiles > g.max_tiles) {
         g.out_log10[pair] = __builtin_nan("");
@@ -372,7 +421,7 @@ __device__ void grad_process(const GradBatch &g, const RescueSink &rs, uint8_t *
     for (int j = lane; j < kCols; j += kWave) {
         L.bm[j] = 0;
         L.bi[j] = 0;
-        L.bs[j] = d0;
+        L.bs[j] = j >= kLdsPad ? d0 : T(0);
     }
     if (R == 0) {  // empty read: log10 L = -inf, nothing to differentiate (LoglessPairHMM.java:62-67)
         g.out_log10[pair] = -__builtin_inf();
@@ -405,15 +454,15 @@ __global__ __launch_bounds__(256) void grad_kernel(GradBatch g, WorkSource ws, R
 
 size_t grad_lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + 5 * (cols + 1) * (fp64 ? sizeof(double) : sizeof(float));
+    return cols + 5 * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))) * (fp64 ? sizeof(double) : sizeof(float));
 }
 
 size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_stripes, int *max_tiles) {
     const int cols = lds_columns(max_hap_len);
-    *max_stripes = (max_read_len + kRows - 1) / kRows;
+    *max_stripes = (max_read_len + kWave - 1) / kWave;
     if (*max_stripes < 1) *max_stripes = 1;
     *max_tiles = dev::round_up(max_hap_len + kWave, kGradTile) / kGradTile;
-    return static_cast<size_t>(*max_stripes) * (static_cast<size_t>(*max_tiles) * 6 * kWave + 3 * static_cast<size_t>(cols));
+    return static_cast<size_t>(*max_stripes) * (static_cast<size_t>(*max_tiles) * 6 * kWave + 2 * static_cast<size_t>(cols));
 }
 
 namespace {

@@ -40,11 +40,8 @@ struct WaveLds {
     T *bm, *bi, *bs;   // [cols]: boundary row M, I and S = I + D, column c at kLdsPad + c
 };
 
-// Boundary planes are kCols + 4 elements apart: 16-byte aligned (lane 0's A / B reads are
-// vector loads) and not a multiple of 64 dwords, so lane 63's three stores stay single
-// ds_write_b32 off one base register rather than ds_write2st64 pairs with their own address adds.
 template <int kCols>
-constexpr int plane_stride() { return kCols + 4; }
+constexpr int plane_stride() { return lds_plane_stride(kCols); }
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() { return static_cast<size_t>(kCols) + 3 * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T); }
@@ -286,7 +283,7 @@ int lds_class_index(int hap_len) {
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + 3 * (cols + 1) * (fp64 ? sizeof(double) : sizeof(float));
+    return cols + 3 * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))) * (fp64 ? sizeof(double) : sizeof(float));
 }
 
 namespace {

@@ -67,6 +67,11 @@ hipError_t dispatch_col_class(int cols, F &&f) {
     }
 }
 
+// Elements between the boundary planes of a wave's LDS slice, host and device alike: 16-byte
+// aligned (vector reads of lane 0's sums) and not a multiple of 64 dwords (single ds_write_b32 off
+// one base register rather than ds_write2st64 pairs).
+constexpr int lds_plane_stride(int cols) { return cols + 4; }
+
 // Bytes of LDS one wavefront needs for haplotypes up to max_hap_len.
 int lds_columns(int max_hap_len);
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64);
