@@ -39,6 +39,8 @@ namespace {
 
 using namespace dev;
 
+constexpr int kCk = 5;  // checkpoint: M, I, D one step back and M, S two steps back
+
 template <typename T>
 struct AdjointScale;
 template <>
@@ -95,16 +97,19 @@ __device__ __forceinline__ Row<T> load_row(const GradBatch &g, int64_t ro, int R
 template <typename T, int kCols>
 struct GradLds {
     uint8_t *hap;
-    T *bm, *bi, *bs;  // forward boundary / relay row: M, I, S by column
-    T *bx, *bg;       // adjoint boundary: X and gI of the row below a stripe, by column
+    T *bm, *bi, *bs;  // forward boundary row M, I, S by column; lane 0's sums A (over S), B (over I)
+    T *bx, *bg;       // adjoint boundary: X and gI of the row below a stripe, by column.  X shares the
+                      // M plane, which the adjoint no longer needs (four planes per wave, not five).
 };
 
 template <int kCols>
 constexpr int grad_stride() { return lds_plane_stride(kCols); }
 
+constexpr int kGradPlanes = 4;
+
 template <typename T, int kCols>
 constexpr size_t grad_lds_per_wave() {
-    return static_cast<size_t>(kCols) + 5 * static_cast<size_t>(grad_stride<kCols>()) * sizeof(T);
+    return static_cast<size_t>(kCols) + kGradPlanes * static_cast<size_t>(grad_stride<kCols>()) * sizeof(T);
 }
 
 template <typename T, int kCols>
@@ -114,8 +119,8 @@ __device__ __forceinline__ GradLds<T, kCols> grad_carve(uint8_t *base) {
     w.bm = reinterpret_cast<T *>(base + kCols);
     w.bi = w.bm + grad_stride<kCols>();
     w.bs = w.bi + grad_stride<kCols>();
-    w.bx = w.bs + grad_stride<kCols>();
-    w.bg = w.bx + grad_stride<kCols>();
+    w.bg = w.bs + grad_stride<kCols>();
+    w.bx = w.bm;
     return w;
 }
 
@@ -186,8 +191,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
     const int n_stripes = (R + kStripe - 1) / kStripe;
     const int steps = round_up(H + kWave, kTile);
     const int n_tiles = steps / kTile;
-    T *ck = scratch;                                                        // [stripe][tile][6][64]
-    T *abp = scratch + static_cast<size_t>(g.max_stripes) * g.max_tiles * 6 * kWave;  // [stripe][2][kCols]
+    T *ck = scratch;                                                        // [stripe][tile][kCk][64]
+    T *abp = scratch + static_cast<size_t>(g.max_stripes) * g.max_tiles * kCk * kWave;  // [stripe][2][kCols]
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
     using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
     lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
@@ -209,15 +214,14 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
         }
         T m1 = 0, i1 = 0, d1 = 0, s1 = 0, m2 = 0, s2 = 0;
         T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *wsp = L.bs + kLdsPad - kLag;
-        T *ckp = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
+        T *ckp = ck + static_cast<size_t>(s) * g.max_tiles * kCk * kWave + lane;
         for (int t0 = 0; t0 < steps; t0 += kTile) {
-            ckp[0 * kWave] = m1;
+            ckp[0 * kWave] = m1;  // S one step back is I + D, recomputed on restore
             ckp[1 * kWave] = i1;
             ckp[2 * kWave] = d1;
-            ckp[3 * kWave] = s1;
-            ckp[4 * kWave] = m2;
-            ckp[5 * kWave] = s2;
-            ckp += 6 * kWave;
+            ckp[3 * kWave] = m2;
+            ckp[4 * kWave] = s2;
+            ckp += kCk * kWave;
             T av[kTile], bv[kTile];
             __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
             __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
@@ -264,7 +268,9 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
     const double norm = weight / (l_scaled * __builtin_ldexp(1.0, sigma) * 2.30258509299404568402);
     const double dq = -0.23025850929940456840;  // d e(q)/dq = dq * e(q)
 
-    // Adjoint boundary below the last stripe: the seed row (gI = seed on columns 1..H, X = 0).
+    // Adjoint boundary below the last stripe: the seed row (gI = seed on columns 1..H, X = 0).  X
+    // shares the M plane that the sum above just read.
+    __builtin_amdgcn_wave_barrier();
     for (int j = lane; j < kCols; j += kWave) {
         const int c = j - kLdsPad;
         L.bx[j] = 0;
@@ -298,13 +304,19 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
         T *bxw = L.bx + kLdsPad, *bgw = L.bg + kLdsPad;
         T X1 = 0, X2 = 0, gI1 = 0, gD1 = 0;
         T aMMn = 0, aGMn = 0, aMIn = 0, aIIn = 0, aMD = 0, aDD = 0, aE = 0;
-        const T *ckbase = ck + static_cast<size_t>(s) * g.max_tiles * 6 * kWave + lane;
+        const T *ckbase = ck + static_cast<size_t>(s) * g.max_tiles * kCk * kWave + lane;
         const bool is_x = lane_x == 0;
+        // Checkpoints are read one tile ahead so the global load overlaps the current tile.
+        T cur[kCk], nxt[kCk];
+#pragma unroll
+        for (int k = 0; k < kCk; ++k) cur[k] = ckbase[(static_cast<size_t>(n_tiles - 1) * kCk + k) * kWave];
         for (int tile = n_tiles - 1; tile >= 0; --tile) {
             const int t0 = tile * kTile;
-            const T *ckp = ckbase + static_cast<size_t>(tile) * 6 * kWave;
-            T m1 = ckp[0 * kWave], i1 = ckp[1 * kWave], d1 = ckp[2 * kWave], s1 = ckp[3 * kWave];
-            T m2 = ckp[4 * kWave], s2 = ckp[5 * kWave];
+            const int prev = tile > 0 ? tile - 1 : 0;
+#pragma unroll
+            for (int k = 0; k < kCk; ++k) nxt[k] = ckbase[(static_cast<size_t>(prev) * kCk + k) * kWave];
+            T m1 = cur[0], i1 = cur[1], d1 = cur[2], m2 = cur[3], s2 = cur[4];
+            T s1 = i1 + d1;
             const T m_before = m1, d_before = d1;
             T av[kTile], bv[kTile];
             __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
@@ -313,11 +325,12 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 #pragma unroll
             for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
             __builtin_amdgcn_wave_barrier();
-            T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stP[kTile], stAW[kTile];
+            T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stAW[kTile];
 #pragma unroll
             for (int u = 0; u < kTile; ++u) {
+                T pr;
                 const T a = fwd_step<T, kHapHasN>(w, px, hbs[u], av[u], bv[u], is_first, m1, i1, d1, s1, m2, s2,
-                                                  stM[u], stI[u], stD[u], stS[u], stP[u]);
+                                                  stM[u], stI[u], stD[u], stS[u], pr);
                 const bool hit = (hbs[u] == w.base) || (kHapHasN && hbs[u] == 'N');
                 const T wgt = (hit || w.n_read) ? T(-1) : T(1) / T(3);  // d prior / d e(qb)
                 stAW[u] = a * wgt;
@@ -331,7 +344,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 const T gD = fma(w.DD, gD1, gS);
                 const T gI = fma(IIn, gIn1, gS);
                 const T gM = fma(w.MD, gD1, fma(MIn, gIn1, MMn * Xn2));
-                const T X = stP[u] * gM;
+                const bool hit = (hbs[u] == w.base) || (kHapHasN && hbs[u] == 'N');
+                const T X = (hit ? w.pm : px) * gM;  // prior recomputed: fewer live registers
                 const T Mp = (u > 0) ? stM[u - 1] : m_before;
                 const T Dp = (u > 0) ? stD[u - 1] : d_before;
                 aMMn = fma(Xn2, stM[u], aMMn);
@@ -350,6 +364,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 gI1 = gI;
                 gD1 = gD;
             }
+#pragma unroll
+            for (int k = 0; k < kCk; ++k) cur[k] = nxt[k];
         }
         __builtin_amdgcn_wave_barrier();
         // The next-row sums belong to the lane below: shift them down one lane.  Lane 63's belong to
@@ -434,7 +450,7 @@ __device__ void grad_process(const GradBatch &g, const RescueSink &rs, uint8_t *
 }
 
 template <typename T, int kCols, int kTile>
-__global__ __launch_bounds__(256) void grad_kernel(GradBatch g, WorkSource ws, RescueSink rs) {
+__device__ __forceinline__ void grad_loop(const GradBatch &g, const WorkSource &ws, const RescueSink &rs) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -450,11 +466,25 @@ __global__ __launch_bounds__(256) void grad_kernel(GradBatch g, WorkSource ws, R
     }
 }
 
+// FP32: held to 168 VGPRs (three waves per SIMD; the few values the allocator then keeps in scratch
+// are reloaded once per stripe, outside the step loops).
+template <int kCols, int kTile>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void grad_kernel_f32(GradBatch g, WorkSource ws,
+                                                                                              RescueSink rs) {
+    grad_loop<float, kCols, kTile>(g, ws, rs);
+}
+
+template <int kCols, int kTile>
+__global__ __launch_bounds__(256) void grad_kernel_f64(GradBatch g, WorkSource ws, RescueSink rs) {
+    grad_loop<double, kCols, kTile>(g, ws, rs);
+}
+
 }  // namespace
 
 size_t grad_lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + 5 * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))) * (fp64 ? sizeof(double) : sizeof(float));
+    return cols + kGradPlanes * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))) *
+                      (fp64 ? sizeof(double) : sizeof(float));
 }
 
 size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_stripes, int *max_tiles) {
@@ -462,7 +492,7 @@ size_t grad_scratch_elems_per_wave(int max_read_len, int max_hap_len, int *max_s
     *max_stripes = (max_read_len + kWave - 1) / kWave;
     if (*max_stripes < 1) *max_stripes = 1;
     *max_tiles = dev::round_up(max_hap_len + kWave, kGradTile) / kGradTile;
-    return static_cast<size_t>(*max_stripes) * (static_cast<size_t>(*max_tiles) * 6 * kWave + 2 * static_cast<size_t>(cols));
+    return static_cast<size_t>(*max_stripes) * (static_cast<size_t>(*max_tiles) * kCk * kWave + 2 * static_cast<size_t>(cols));
 }
 
 namespace {
@@ -476,7 +506,7 @@ hipError_t with_grad_kernel(bool fp64, int cols, F &&f) {
         if constexpr (kC > kMaxGradCols) {
             return hipErrorInvalidValue;
         } else {
-            return fp64 ? f(grad_kernel<double, kC, kGradTile>) : f(grad_kernel<float, kC, kGradTile>);
+            return fp64 ? f(grad_kernel_f64<kC, kGradTile>) : f(grad_kernel_f32<kC, kGradTile>);
         }
     });
 }
