@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_CELL = 12          # M: 6, I: 3, D: 3 (LoglessPairHMM.java:51-55)
 FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published)
+# HBM bytes per cell of the forward pass: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the FP32 and
+# FP64 forward kernels on 1,000 batched_uneven regions, each scaled by its reading for a 1 GiB
+# elementwise copy (x2.0 and x1.0); the rocprof summary is in profiles/.
+HBM_BYTES_PER_CELL = 0.0081
 
 
 def log(msg: str) -> None:
@@ -141,6 +145,21 @@ def cpu_grad_baseline(ctx, threads, n_reads=2):
             "max_abs_dlog10L_sample": worst_l, "grad_rel_err_sample": worst_g}
 
 
+def streaming_leg(args):
+    """Regions streamed through the double-buffered stream API in chunks generated by worker processes."""
+    from gatk_amd.stream import run_streaming
+
+    workers = min(16, os.cpu_count() or 1)
+    res = run_streaming(args.stream_regions, chunk_regions=args.stream_chunk, workers=workers, log=log)
+    d = {"regions": res.regions, "pairs": res.pairs, "cells": res.cells, "wall_s": round(res.wall_s, 2),
+         "gcups": round(res.gcups, 1), "rescued_pairs": res.rescued, "chunk_regions": args.stream_chunk,
+         "generator_workers": workers,
+         "workload": "90% batched_uneven-shaped regions, 10% precision_stress (STR-rich) regions, "
+                     "double-buffered host queue, wall time includes chunk hand-over, packing and H2D/D2H"}
+    log(f"streaming: {d}")
+    return d
+
+
 def str_fit_leg(args):
     """Per-(STR period, repeats) GOP / GCP fit with the adjoint kernel against known penalties."""
     from gatk_amd import strfit
@@ -174,6 +193,9 @@ def main() -> int:
                     help="regions of the STR penalty fit leg (100 reads each; 0 skips it)")
     ap.add_argument("--fit-iters", type=int, default=200)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-schedule autotuner")
+    ap.add_argument("--stream-regions", type=int, default=20_000,
+                    help="regions of the streaming leg (90%% batched_uneven, 10%% STR-rich; 0 skips it)")
+    ap.add_argument("--stream-chunk", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -192,6 +214,11 @@ def main() -> int:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (regions shard across GPUs "
               f"with no exchange; multi-GPU runs are out of scope here)", file=sys.stderr)
         return 2
+
+    # Streaming leg first: its generator pool starts before this process touches the GPU.
+    stream = None
+    if args.stream_regions > 0:
+        stream = streaming_leg(args)
 
     from gatk_amd import _native, synth
     from gatk_amd.batch import BatchEngine
@@ -299,14 +326,16 @@ def main() -> int:
         "fwd_bwd_gcups": bwd["gcups"] if bwd else None,
         "fwd_bwd": bwd,
         "str_fit": fit,
+        "streaming": stream,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
         "max_abs_dlog10L_vs_fp64_sample": worst,
         "checked_pairs": n_checked,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "forward_kernel<float>", "kernel_ms": round(st.fp32_kernel_ms, 3),
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": int(HBM_BYTES_PER_CELL * cells),
+                     "kernel": "forward_kernel<float, C> over its LDS-class launches of one pass",
+                     "kernel_ms": round(st.fp32_kernel_ms, 3),
                      "flop_per_cell": FLOP_PER_CELL},
         "cpu_baseline": cpu,
     }

@@ -325,12 +325,11 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 #pragma unroll
             for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
             __builtin_amdgcn_wave_barrier();
-            T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stAW[kTile];
+            T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stP[kTile], stAW[kTile];
 #pragma unroll
             for (int u = 0; u < kTile; ++u) {
-                T pr;
                 const T a = fwd_step<T, kHapHasN>(w, px, hbs[u], av[u], bv[u], is_first, m1, i1, d1, s1, m2, s2,
-                                                  stM[u], stI[u], stD[u], stS[u], pr);
+                                                  stM[u], stI[u], stD[u], stS[u], stP[u]);
                 const bool hit = (hbs[u] == w.base) || (kHapHasN && hbs[u] == 'N');
                 const T wgt = (hit || w.n_read) ? T(-1) : T(1) / T(3);  // d prior / d e(qb)
                 stAW[u] = a * wgt;
@@ -344,8 +343,7 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 const T gD = fma(w.DD, gD1, gS);
                 const T gI = fma(IIn, gIn1, gS);
                 const T gM = fma(w.MD, gD1, fma(MIn, gIn1, MMn * Xn2));
-                const bool hit = (hbs[u] == w.base) || (kHapHasN && hbs[u] == 'N');
-                const T X = (hit ? w.pm : px) * gM;  // prior recomputed: fewer live registers
+                const T X = stP[u] * gM;
                 const T Mp = (u > 0) ? stM[u - 1] : m_before;
                 const T Dp = (u > 0) ? stD[u - 1] : d_before;
                 aMMn = fma(Xn2, stM[u], aMMn);
