@@ -160,6 +160,47 @@ def streaming_leg(args):
     return d
 
 
+def chunked_leg(args, chunk_regions=500):
+    """The streaming workload's region mix (90% batched_uneven-shaped, 10% STR-rich precision_stress
+    regions), generated chunk by chunk in this process and run through the batch API (upload +
+    persistent graph + download per chunk).  GPU-side GCUPS counts the forward launches only; the
+    wall figure adds the upload and download of every chunk (generation excluded)."""
+    from gatk_amd import _native
+    from gatk_amd.batch import BatchEngine
+    from gatk_amd.stream import _make_chunk
+
+    n = args.chunked_regions
+    cells = pairs = rescued = 0
+    gpu_ms = 0.0
+    wall = 0.0
+    gen = 0.0
+    eng = BatchEngine()
+    for k in range((n + chunk_regions - 1) // chunk_regions):
+        t0 = time.perf_counter()
+        chunk = _make_chunk((min(chunk_regions, n - k * chunk_regions), 4_000_000 + k, 0.10))
+        t1 = time.perf_counter()
+        dev = eng.upload(chunk)
+        out = np.empty(chunk.n_pairs)
+        st = dev.forward(mode=_native.RUN_PERSISTENT, out=out)
+        dev.free()
+        t2 = time.perf_counter()
+        gen += t1 - t0
+        wall += t2 - t1
+        gpu_ms += st.total_ms
+        cells += st.cells
+        pairs += chunk.n_pairs
+        rescued += st.n_rescued
+        if (k + 1) % 5 == 0:
+            log(f"chunked: {k + 1} chunks, {pairs} pairs, GPU {gpu_ms / 1e3:.2f} s")
+    eng.close()
+    d = {"regions": n, "chunk_regions": chunk_regions, "pairs": pairs, "cells": cells, "rescued_pairs": rescued,
+         "gpu_gcups": round(cells / (gpu_ms * 1e-3) / 1e9, 1), "wall_gcups_with_transfers": round(cells / wall / 1e9, 1),
+         "generation_s": round(gen, 1),
+         "workload": "streaming mix: 90% batched_uneven-shaped, 10% precision_stress regions; batch API per chunk"}
+    log(f"chunked streaming workload: {d}")
+    return d
+
+
 def str_fit_leg(args):
     """Per-(STR period, repeats) GOP / GCP fit with the adjoint kernel against known penalties."""
     from gatk_amd import strfit
@@ -193,9 +234,13 @@ def main() -> int:
                     help="regions of the STR penalty fit leg (100 reads each; 0 skips it)")
     ap.add_argument("--fit-iters", type=int, default=200)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-schedule autotuner")
-    ap.add_argument("--stream-regions", type=int, default=20_000,
-                    help="regions of the streaming leg (90%% batched_uneven, 10%% STR-rich; 0 skips it)")
+    ap.add_argument("--stream-regions", type=int, default=0,
+                    help="regions of the double-buffered stream-API leg with a generator process pool "
+                         "(off by default; see the streaming notes of the design document)")
     ap.add_argument("--stream-chunk", type=int, default=500)
+    ap.add_argument("--chunked-regions", type=int, default=10_000,
+                    help="regions of the chunked streaming-workload leg (90%% batched_uneven, 10%% STR-rich; "
+                         "0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -298,6 +343,9 @@ def main() -> int:
     fit = None
     if args.fit_regions > 0:
         fit = str_fit_leg(args)
+    chunked = None
+    if args.chunked_regions > 0:
+        chunked = chunked_leg(args)
     cpu, worst, n_checked = None, None, 0
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -327,6 +375,7 @@ def main() -> int:
         "fwd_bwd": bwd,
         "str_fit": fit,
         "streaming": stream,
+        "streaming_workload_chunked": chunked,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
