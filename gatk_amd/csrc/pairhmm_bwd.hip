@@ -477,6 +477,51 @@ __global__ __launch_bounds__(256) void grad_kernel_f64(GradBatch g, WorkSource w
     grad_loop<double, kCols, kTile>(g, ws, rs);
 }
 
+// LDS column class of a haplotype length (the same rule as lds_class_index), capped at max_class.
+__device__ __forceinline__ int hap_class(int H, int max_class) {
+    const int need = H + 2 * kLdsPad + 32;
+    int k = max_class;
+#pragma unroll
+    for (int c = kNumColClasses - 1; c >= 0; --c)
+        if (kColClasses[c] >= need) k = c;
+    return k < max_class ? k : max_class;
+}
+
+__global__ __launch_bounds__(256) void grad_class_count_kernel(const int32_t *pair_hap, const int32_t *hap_len,
+                                                               uint32_t n_pairs, int max_class, uint32_t *counts) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int k = p < n_pairs ? hap_class(hap_len[pair_hap[p]], max_class) : -1;
+#pragma unroll
+    for (int c = 0; c < kNumColClasses; ++c) {  // one atomic per class present in the wave
+        const uint64_t m = __ballot(k == c);
+        if (m != 0 && lane == __builtin_ctzll(m)) atomicAdd(&counts[c], static_cast<uint32_t>(__builtin_popcountll(m)));
+    }
+}
+
+__global__ __launch_bounds__(256) void grad_class_scatter_kernel(const int32_t *pair_hap, const int32_t *hap_len,
+                                                                 uint32_t n_pairs, int max_class, ClassRuns runs,
+                                                                 uint32_t *cursors, uint32_t *order) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int k = p < n_pairs ? hap_class(hap_len[pair_hap[p]], max_class) : -1;
+    int run = -1;
+#pragma unroll
+    for (int c = 0; c < kNumColClasses; ++c)
+        if (k == c) run = runs.run_of_class[c];
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));  // lanes < this one
+#pragma unroll
+    for (int r = 0; r < kNumColClasses; ++r) {
+        const uint64_t m = __ballot(run == r);
+        if (m == 0) continue;
+        const int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cursors[r], static_cast<uint32_t>(__builtin_popcountll(m)));
+        base = __shfl(base, leader, kWave);
+        if (run == r) order[runs.run_begin[r] + base + static_cast<uint32_t>(__builtin_popcountll(m & below))] = p;
+    }
+}
+
 }  // namespace
 
 size_t grad_lds_bytes_per_wave(int max_hap_len, bool fp64) {
@@ -529,6 +574,24 @@ LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block
     });
     sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
     return sh;
+}
+
+hipError_t launch_grad_class_count(const int32_t *pair_hap, const int32_t *hap_len, uint32_t n_pairs, int max_class,
+                                   uint32_t *counts, hipStream_t stream) {
+    if (n_pairs == 0) return hipSuccess;
+    if (max_class < 0 || max_class >= kNumColClasses) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(grad_class_count_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0, stream, pair_hap, hap_len,
+                       n_pairs, max_class, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_class_scatter(const int32_t *pair_hap, const int32_t *hap_len, uint32_t n_pairs, int max_class,
+                                     const ClassRuns &runs, uint32_t *cursors, uint32_t *order, hipStream_t stream) {
+    if (n_pairs == 0) return hipSuccess;
+    if (max_class < 0 || max_class >= kNumColClasses) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(grad_class_scatter_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0, stream, pair_hap, hap_len,
+                       n_pairs, max_class, runs, cursors, order);
+    return hipGetLastError();
 }
 
 hipError_t launch_grad(bool fp64, const GradBatch &g, const WorkSource &work, const RescueSink &rescue, int max_hap_len,

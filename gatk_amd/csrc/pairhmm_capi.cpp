@@ -563,23 +563,88 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
         !d->grad_qb || !d->grad_qi || !d->grad_qd || !d->grad_qc)
         return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
     HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
-    const LaunchShape sh32 = grad_shape(false, d->max_hap_len, 4);
+    const LaunchShape sh32max = grad_shape(false, d->max_hap_len, 4);
     const LaunchShape sh64 = grad_shape(true, d->max_hap_len, 4);
-    if (sh32.waves_per_block == 0 || sh64.waves_per_block == 0)
+    if (sh32max.waves_per_block == 0 || sh64.waves_per_block == 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(d->max_hap_len) +
                                              " bases exceeds the adjoint kernel's LDS budget");
+    const int max_class = lds_class_index(d->max_hap_len);
+    const uint32_t np = static_cast<uint32_t>(d->n_pairs);
+    // Work area: 64 counter slots ([1] FP64 item count, [2] FP64 head, [16+k] pairs of class k,
+    // [32+r] scatter cursor of run r, [48+r] queue head of run r), the FP64 list, the class order.
+    if (ctx->grad_work.reserve(sizeof(uint32_t) * (64 + 2 * static_cast<size_t>(np))) != hipSuccess)
+        return fail(PAIRHMM_ERR_OOM, "adjoint work-list allocation failed");
+    uint32_t *counters = static_cast<uint32_t *>(ctx->grad_work.ptr);
+    uint32_t *rescue = counters + 64;
+    uint32_t *order = rescue + np;
+    hipStream_t s = ctx->stream;
+    hipEvent_t ev[3] = {};
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_zero_counters(counters, 64, s), PAIRHMM_ERR_KERNEL);
+
+    // FP32 launches: one per run of LDS classes, largest class first; classes with few pairs merge
+    // into the next larger class present (the forward's queue slices, binned on the device here
+    // because the pair table is device-resident).
+    struct Run {
+        int cls;
+        uint32_t begin, count;
+        LaunchShape shape;
+        int grid;
+    };
+    std::vector<Run> runs;
+    ClassRuns cr{};
+    if (!d->use_double) {
+        HIP_TRY(launch_grad_class_count(d->pair_hap, d->hap_len, np, max_class, counters + 16, s), PAIRHMM_ERR_KERNEL);
+        uint32_t counts[kNumColClasses] = {};
+        HIP_TRY(hipMemcpyAsync(counts, counters + 16, sizeof(counts), hipMemcpyDeviceToHost, s), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(hipStreamSynchronize(s), PAIRHMM_ERR_KERNEL);
+        const uint32_t merge_below = std::max<uint32_t>(4096, np / 64);
+        uint32_t pending = 0;
+        std::vector<int> members;
+        for (int k = 0; k < kNumColClasses; ++k) cr.run_of_class[k] = -1;
+        for (int k = 0; k <= max_class; ++k) {
+            if (counts[k] == 0) continue;
+            pending += counts[k];
+            members.push_back(k);
+            bool later = false;
+            for (int j = k + 1; j <= max_class; ++j) later |= counts[j] > 0;
+            if (pending >= merge_below || !later) {
+                for (int m : members) cr.run_of_class[m] = static_cast<int32_t>(runs.size());
+                runs.push_back(Run{k, 0, pending, LaunchShape{}, 0});
+                pending = 0;
+                members.clear();
+            }
+        }
+        uint32_t acc = 0;  // largest class first in the order list
+        for (size_t r = runs.size(); r-- > 0;) {
+            runs[r].begin = acc;
+            cr.run_begin[r] = acc;
+            acc += runs[r].count;
+            const int cap = kColClasses[runs[r].cls] - 2 * kLdsPad - 32;  // longest haplotype of the class
+            runs[r].shape = grad_shape(false, std::min(cap, d->max_hap_len), 4);
+            if (runs[r].shape.waves_per_block == 0) return fail(PAIRHMM_ERR_KERNEL, "adjoint class shape");
+            runs[r].grid = persistent_grid(ctx, runs[r].shape, runs[r].count);
+        }
+        HIP_TRY(launch_grad_class_scatter(d->pair_hap, d->hap_len, np, max_class, cr, counters + 32, order, s),
+                PAIRHMM_ERR_KERNEL);
+    }
     int max_stripes = 0, max_tiles = 0;
     const size_t per_wave = grad_scratch_elems_per_wave(d->max_read_len, d->max_hap_len, &max_stripes, &max_tiles);
     const int cus = std::max(ctx->num_cus, 1);
-    const int grid32 = sh32.blocks_per_cu * cus, grid64 = sh64.blocks_per_cu * cus;
-    const size_t waves32 = static_cast<size_t>(grid32) * sh32.waves_per_block;
+    const int grid64 = sh64.blocks_per_cu * cus;
+    size_t waves32 = 0;
+    for (const Run &r : runs) waves32 = std::max(waves32, static_cast<size_t>(r.grid) * r.shape.waves_per_block);
     const size_t waves64 = static_cast<size_t>(grid64) * sh64.waves_per_block;
-    if (ctx->grad_scratch32.reserve(waves32 * per_wave * sizeof(float)) != hipSuccess ||
-        ctx->grad_scratch64.reserve(waves64 * per_wave * sizeof(double)) != hipSuccess ||
-        ctx->grad_work.reserve(64 + sizeof(uint32_t) * static_cast<size_t>(d->n_pairs)) != hipSuccess)
+    if (ctx->grad_scratch32.reserve(std::max<size_t>(waves32, 1) * per_wave * sizeof(float)) != hipSuccess ||
+        ctx->grad_scratch64.reserve(waves64 * per_wave * sizeof(double)) != hipSuccess)
         return fail(PAIRHMM_ERR_OOM, "adjoint scratch allocation failed");
-    uint32_t *counters = static_cast<uint32_t *>(ctx->grad_work.ptr);  // [0] head32 [1] rescued [2] head64
-    uint32_t *rescue = counters + 16;
     GradBatch g{};
     g.hap_bases = d->hap_bases;
     g.hap_off = d->hap_off;
@@ -603,23 +668,12 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     g.scratch_per_wave = per_wave;
     g.max_stripes = max_stripes;
     g.max_tiles = max_tiles;
-    hipStream_t s = ctx->stream;
-    hipEvent_t ev[3] = {};
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
-    struct EvGuard {
-        hipEvent_t *e;
-        ~EvGuard() {
-            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
-        }
-    } guard{ev};
     const size_t gbytes = sizeof(double) * static_cast<size_t>(d->read_bytes);
     HIP_TRY(hipMemsetAsync(d->grad_qb, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipMemsetAsync(d->grad_qi, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipMemsetAsync(d->grad_qd, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipMemsetAsync(d->grad_qc, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
     if (d->grad_base_error) HIP_TRY(hipMemsetAsync(d->grad_base_error, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(launch_zero_counters(counters, 16, s), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
     RescueSink none{nullptr, nullptr, 0.0};
     if (d->use_double) {
         HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
@@ -628,9 +682,14 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
         HIP_TRY(launch_grad(true, g, all, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
     } else {
         g.scratch = ctx->grad_scratch32.ptr;
-        WorkSource all{nullptr, static_cast<uint32_t>(d->n_pairs), nullptr, counters + 0};
         RescueSink rs{rescue, counters + 1, 0.0};
-        HIP_TRY(launch_grad(false, g, all, rs, d->max_hap_len, sh32, grid32, s), PAIRHMM_ERR_KERNEL);
+        for (size_t r = runs.size(); r-- > 0;) {
+            const Run &q = runs[r];
+            WorkSource w{order + q.begin, q.count, nullptr, counters + 48 + r};
+            HIP_TRY(launch_grad(false, g, w, rs, std::min(kColClasses[q.cls] - 2 * kLdsPad - 32, d->max_hap_len),
+                                q.shape, q.grid, s),
+                    PAIRHMM_ERR_KERNEL);
+        }
         HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
         g.scratch = ctx->grad_scratch64.ptr;
         WorkSource again{rescue, 0, counters + 1, counters + 2};

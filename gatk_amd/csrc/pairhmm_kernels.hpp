@@ -129,4 +129,18 @@ LaunchShape grad_shape(bool fp64, int max_hap_len, int requested_waves_per_block
 hipError_t launch_grad(bool fp64, const GradBatch &g, const WorkSource &work, const RescueSink &rescue, int max_hap_len,
                        const LaunchShape &shape, int grid_blocks, hipStream_t stream);
 
+// Device-side binning of the adjoint's pairs by LDS column class, so that each adjoint launch gets
+// the LDS slice (and so the occupancy) of its own haplotypes rather than the batch's longest.
+// Classes above max_class count as max_class.  Counting adds the per-class pair counts to counts[];
+// scattering places each pair at run_begin[run] + an atomic cursor of its run, where a run is one
+// launch covering one or more merged classes.
+struct ClassRuns {
+    int32_t run_of_class[kNumColClasses];
+    uint32_t run_begin[kNumColClasses];
+};
+hipError_t launch_grad_class_count(const int32_t *pair_hap, const int32_t *hap_len, uint32_t n_pairs, int max_class,
+                                   uint32_t *counts, hipStream_t stream);
+hipError_t launch_grad_class_scatter(const int32_t *pair_hap, const int32_t *hap_len, uint32_t n_pairs, int max_class,
+                                     const ClassRuns &runs, uint32_t *cursors, uint32_t *order, hipStream_t stream);
+
 }  // namespace gatk_pairhmm
