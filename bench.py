@@ -201,6 +201,58 @@ def chunked_leg(args, chunk_regions=500):
     return d
 
 
+def precision_leg(args, threads):
+    """Precision stress: STR-rich regions with haplotypes of 800-1,500 bp, reads of 250-600 bp, base
+    Q 2-15 and 5% substitutions, so that many pairs fall below the FP32 underflow threshold.  Reports
+    the rescue fraction, the FP32 kernel's and the FP64 rescue's time and throughput, and |d log10 L|
+    against the FP64 oracle on a sample of FP32-kept and of rescued pairs."""
+    from gatk_amd import _native, synth
+    from gatk_amd.batch import BatchEngine
+    from oracle import oracle
+
+    b = synth.workload("precision_stress", n_regions=args.precision_regions, seed=2)
+    cells = b.cells()
+    log(f"precision stress: {args.precision_regions} regions, {b.n_pairs} pairs, {cells:.4e} cells")
+    with BatchEngine() as eng:
+        dev = eng.upload(b)
+        out = np.empty(b.n_pairs)
+        dev.forward(out=out)  # graph capture + warm-up
+        t0 = time.perf_counter()
+        st = dev.forward(out=out)
+        wall = time.perf_counter() - t0
+        ste = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+        dev.free()
+    # A pair is rescued when its FP32 scaled sum L * 2^120 is below 1e-28.
+    rescued = out + 120 * np.log10(2.0) < -28.0
+    r_idx, k_idx = np.nonzero(rescued)[0], np.nonzero(~rescued & np.isfinite(out))[0]
+    r_cells = b.cells(r_idx)
+    rng = np.random.default_rng(5)
+    pick = lambda idx, n: np.sort(rng.choice(idx, size=min(n, len(idx)), replace=False))  # noqa: E731
+    sk, sr = pick(k_idx, args.precision_sample), pick(r_idx, args.precision_sample)
+    t1 = time.perf_counter()
+    ref = oracle.logless_batch(b, n_threads=threads, pairs=np.concatenate([sk, sr]))
+    el = time.perf_counter() - t1
+    err_k = np.abs(out[sk] - ref[:len(sk)])
+    err_r = np.abs(out[sr] - ref[len(sk):])
+    hist_edges = [0, 1e-9, 1e-8, 1e-7, 1e-6, 1e-5, 1e-4, np.inf]
+    hist = lambda e: np.histogram(e, bins=hist_edges)[0].tolist()  # noqa: E731
+    d = {"regions": args.precision_regions, "pairs": b.n_pairs, "cells": cells,
+         "rescued_pairs": int(ste.n_rescued), "rescue_fraction": round(ste.n_rescued / b.n_pairs, 4),
+         "rescued_cells": int(r_cells), "gcups": round(cells / (st.total_ms * 1e-3) / 1e9, 1),
+         "wall_gcups": round(cells / wall / 1e9, 1),
+         "fp32_kernel_ms": round(ste.fp32_kernel_ms, 1), "fp64_rescue_ms": round(ste.rescue_kernel_ms, 1),
+         "fp64_rescue_gcups": round(r_cells / (ste.rescue_kernel_ms * 1e-3) / 1e9, 1) if ste.rescue_kernel_ms > 0 else None,
+         "max_abs_dlog10L_fp32_kept": float(err_k.max()) if len(err_k) else None,
+         "max_abs_dlog10L_fp64_rescued": float(err_r.max()) if len(err_r) else None,
+         "error_histogram_edges": ["0", "1e-9", "1e-8", "1e-7", "1e-6", "1e-5", "1e-4", "inf"],
+         "error_histogram_fp32_kept": hist(err_k), "error_histogram_fp64_rescued": hist(err_r),
+         "oracle_sample": f"{len(sk)} kept + {len(sr)} rescued random pairs, FP64 oracle on {threads} threads, {el:.1f} s",
+         "workload": "precision_stress (seed 2): STR-rich haplotypes 800-1,500 bp, reads 250-600 bp, base Q 2-15, "
+                     "5% substitutions, batched_uneven haplotype/read counts"}
+    log(f"precision stress: {d}")
+    return d
+
+
 def str_fit_leg(args):
     """Per-(STR period, repeats) GOP / GCP fit with the adjoint kernel against known penalties."""
     from gatk_amd import strfit
@@ -241,6 +293,10 @@ def main() -> int:
     ap.add_argument("--chunked-regions", type=int, default=10_000,
                     help="regions of the chunked streaming-workload leg (90%% batched_uneven, 10%% STR-rich; "
                          "0 skips it)")
+    ap.add_argument("--precision-regions", type=int, default=2_000,
+                    help="regions of the precision-stress leg (0 skips it)")
+    ap.add_argument("--precision-sample", type=int, default=60,
+                    help="FP32-kept and rescued pairs each checked against the FP64 oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -346,6 +402,9 @@ def main() -> int:
     chunked = None
     if args.chunked_regions > 0:
         chunked = chunked_leg(args)
+    precision = None
+    if args.precision_regions > 0:
+        precision = precision_leg(args, min(16, os.cpu_count() or 1))
     cpu, worst, n_checked = None, None, 0
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -376,6 +435,7 @@ def main() -> int:
         "str_fit": fit,
         "streaming": stream,
         "streaming_workload_chunked": chunked,
+        "precision_stress": precision,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,

@@ -110,10 +110,20 @@ struct QueueSlice {
     int64_t begin = 0, count = 0;
     int max_hap_len = 1;
 };
-constexpr int kCounterSlots = 16;  // [0] head of slice 0, [1] rescue count, [2] rescue head, [2+k] head of slice k
-constexpr int kMaxSlices = kCounterSlots - 2;
+// Device counters of a batch pass, per queue slice k: [k] FP32 queue head, [16 + k] pairs the slice
+// sent to the FP64 rescue (its rescue list sits at the slice's own offset in the rescue array),
+// [32 + k] FP64 queue head.
+constexpr int kMaxSlices = 16;
+constexpr int kCounterSlots = 4 * kMaxSlices;
 
-inline uint32_t *slice_head(uint32_t *counters, size_t k) { return counters + (k == 0 ? 0 : 2 + k); }
+inline uint32_t *slice_head(uint32_t *counters, size_t k) { return counters + k; }
+inline uint32_t *slice_rescue_count(uint32_t *counters, size_t k) { return counters + kMaxSlices + k; }
+inline uint32_t *slice_rescue_head(uint32_t *counters, size_t k) { return counters + 2 * kMaxSlices + k; }
+inline uint32_t rescued_total(const uint32_t *host_counters) {
+    uint32_t n = 0;
+    for (int k = 0; k < kMaxSlices; ++k) n += host_counters[kMaxSlices + k];
+    return n;
+}
 
 // Validates a host batch descriptor and builds its work queue and slices.  Fills cells (sum of
 // R*H) and the longest haplotype.
@@ -147,10 +157,17 @@ namespace capi {
 int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_t n_items);
 
 // Enqueues the FP32 forward kernel once per queue slice on `s` (counters must be zeroed first);
-// pairs below the rescue threshold are appended to rescue_list.
+// pairs below the rescue threshold are appended to their slice's part of rescue_list (the slice's
+// offset and capacity in the queue).
 hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
                               uint32_t *rescue_list, hipStream_t s);
+
+// Enqueues the FP64 forward kernel once per slice, each with its own slice's LDS class: over the
+// slice's rescue list (rescue_pass, the count read on the device), or over all of its queue.
+hipError_t launch_fp64_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
+                              const uint32_t *list, const std::vector<QueueSlice> &slices, uint32_t *counters,
+                              bool rescue_pass, hipStream_t s);
 
 }  // namespace capi
 }  // namespace gatk_pairhmm

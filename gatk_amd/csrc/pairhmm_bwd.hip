@@ -28,8 +28,9 @@
 //   (d L / d M(R,c) = d L / d I(R,c) = 1) enters as a virtual row R+1 with gI = 1, X = 0 and
 //   MM = GM = 0, MI = II = 1, which also makes rows beyond R pass the seed up unchanged.
 //   Scaling: forward cells carry 2^120 (FP32) / 2^1020 (FP64); the FP32 adjoint seed is 2^sigma with
-//   sigma = 90 - ilogb(L_scaled) so every accumulated product stays inside FP32's range; pairs whose
-//   L_scaled is below 2^-36 go to the FP64 kernel.  Per-lane sums are FP32 (FP64 in the FP64 kernel);
+//   sigma = 30 - ilogb(L_scaled) so every accumulated product stays inside FP32's range; pairs whose
+//   L_scaled is below the forward's rescue threshold (1e-28) go to the FP64 kernel, as in the
+//   forward.  Per-lane sums are FP32 (FP64 in the FP64 kernel);
 //   per-position results are scaled and accumulated in FP64 with atomics.
 #include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
@@ -45,7 +46,11 @@ template <typename T>
 struct AdjointScale;
 template <>
 struct AdjointScale<float> {
-    static constexpr int kTarget = 90;     // L_scaled * 2^sigma ~ 2^90
+    // L_scaled * 2^sigma ~ 2^30.  On the likely path f * b ~ L_scaled, so the adjoint b * 2^sigma
+    // is about 2^(30 - 111) where the path starts (f ~ 2^120 / H) and the seed 2^sigma at its end;
+    // with the seed held below 2^126 this covers every pair down to L_scaled ~ 2^-96, i.e. every
+    // pair the FP32 forward keeps (L_scaled >= 1e-28 ~ 2^-93).
+    static constexpr int kTarget = 30;
     static constexpr int kMaxSigma = 126;  // seed must stay below FLT_MAX
 };
 template <>
@@ -253,7 +258,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
     int sigma = 0;
     if (AdjointScale<T>::kMaxSigma > 0) {
         sigma = AdjointScale<T>::kTarget - ilogb(l_scaled);
-        const int bad = __builtin_amdgcn_readfirstlane(static_cast<int>(!(l_scaled > 0.0) || sigma > AdjointScale<T>::kMaxSigma));
+        const int bad = __builtin_amdgcn_readfirstlane(
+            static_cast<int>(!(l_scaled > 0.0) || sigma > AdjointScale<T>::kMaxSigma || l_scaled < rs.threshold));
         if (bad) {  // hand the pair to the FP64 kernel
             if (rs.list != nullptr) {
                 const uint32_t slot = __builtin_amdgcn_readfirstlane(atomicAdd(rs.count, lane == 0 ? 1u : 0u));

@@ -28,9 +28,7 @@ struct pairhmm_batch {
     uint32_t *d_rescue = nullptr;   // rescue list (n_pairs capacity)
     uint32_t *d_counters = nullptr; // kCounterSlots: queue heads and rescue counters
     hipGraphExec_t graph = nullptr;
-    std::vector<QueueSlice> slices; // FP32 launches over the queue
-    LaunchShape sh64;               // FP64 rescue launch
-    int fp64_grid = 0;
+    std::vector<QueueSlice> slices; // FP32 and FP64 launches over the queue, one per LDS class run
 };
 
 extern "C" {
@@ -344,13 +342,34 @@ int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_
 hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
                               uint32_t *rescue_list, hipStream_t s) {
-    RescueSink rs{rescue_list, counters + 1, kFp32RescueThreshold};
+    if (slices.size() > static_cast<size_t>(kMaxSlices)) return hipErrorInvalidValue;
     for (size_t k = 0; k < slices.size(); ++k) {
         const QueueSlice &q = slices[k];
         const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block);
         if (sh.waves_per_block == 0) return hipErrorInvalidValue;
+        RescueSink rs{rescue_list + q.begin, slice_rescue_count(counters, k), kFp32RescueThreshold};
         WorkSource w{order + q.begin, static_cast<uint32_t>(q.count), nullptr, slice_head(counters, k)};
         const hipError_t e = launch_forward(false, batch, tab, w, rs, q.max_hap_len, sh,
+                                            persistent_grid(ctx, sh, q.count), s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_fp64_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
+                              const uint32_t *list, const std::vector<QueueSlice> &slices, uint32_t *counters,
+                              bool rescue_pass, hipStream_t s) {
+    if (slices.size() > static_cast<size_t>(kMaxSlices)) return hipErrorInvalidValue;
+    RescueSink none{nullptr, nullptr, 0.0};
+    for (size_t k = 0; k < slices.size(); ++k) {
+        const QueueSlice &q = slices[k];
+        const LaunchShape sh = forward_shape(true, q.max_hap_len, ctx->waves_per_block);
+        if (sh.waves_per_block == 0) return hipErrorInvalidValue;
+        // A rescue pass reads its item count on the device; the grid is sized for the slice's
+        // capacity and surplus waves leave at their first queue fetch.
+        WorkSource w = rescue_pass ? WorkSource{list + q.begin, 0, slice_rescue_count(counters, k), slice_rescue_head(counters, k)}
+                                   : WorkSource{list + q.begin, static_cast<uint32_t>(q.count), nullptr, slice_head(counters, k)};
+        const hipError_t e = launch_forward(true, batch, tab, w, none, q.max_hap_len, sh,
                                             persistent_grid(ctx, sh, q.count), s);
         if (e != hipSuccess) return e;
     }
@@ -431,12 +450,11 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     bt->d_rescue = reinterpret_cast<uint32_t *>(m + o_res);
     bt->d_counters = reinterpret_cast<uint32_t *>(m + o_cnt);
     bt->slices = std::move(slices);
-    bt->sh64 = forward_shape(true, max_h, ctx->waves_per_block);
-    if (forward_shape(false, max_h, ctx->waves_per_block).waves_per_block == 0 || bt->sh64.waves_per_block == 0) {
+    if (forward_shape(false, max_h, ctx->waves_per_block).waves_per_block == 0 ||
+        forward_shape(true, max_h, ctx->waves_per_block).waves_per_block == 0) {
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     }
-    bt->fp64_grid = persistent_grid(ctx, bt->sh64, np);
     *out = bt;
     return PAIRHMM_OK;
 }
@@ -454,9 +472,7 @@ hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
     e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
-    WorkSource q2{bt->d_rescue, 0, bt->d_counters + 1, bt->d_counters + 2};
-    RescueSink none{nullptr, nullptr, 0.0};
-    e = launch_forward(true, bt->view, tab, q2, none, bt->max_hap_len, bt->sh64, bt->fp64_grid, s);
+    e = launch_fp64_slices(ctx, bt->view, tab, bt->d_rescue, bt->slices, bt->d_counters, true, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     return e;
 }
@@ -506,15 +522,13 @@ extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt
         DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
         HIP_TRY(launch_zero_counters(bt->d_counters, kCounterSlots, s), PAIRHMM_ERR_KERNEL);
         HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
-        WorkSource q{bt->d_order, static_cast<uint32_t>(bt->n_pairs), nullptr, bt->d_counters + 0};
-        RescueSink none{nullptr, nullptr, 0.0};
-        HIP_TRY(launch_forward(true, bt->view, tab, q, none, bt->max_hap_len, bt->sh64, bt->fp64_grid, s),
+        HIP_TRY(launch_fp64_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, false, s),
                 PAIRHMM_ERR_KERNEL);
         HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
     } else {
         return fail(PAIRHMM_ERR_BAD_ARG, "unknown run mode");
     }
-    uint32_t counters[4] = {};
+    uint32_t counters[kCounterSlots] = {};
     HIP_TRY(hipMemcpyAsync(counters, bt->d_counters, sizeof(counters), hipMemcpyDeviceToHost, s), PAIRHMM_ERR_KERNEL);
     if (out_host)
         HIP_TRY(hipMemcpyAsync(out_host, bt->view.out, sizeof(double) * bt->n_pairs, hipMemcpyDeviceToHost, s),
@@ -526,7 +540,7 @@ extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt
         if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.fp32_kernel_ms = ms;
         if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
     }
-    st.n_rescued = (mode == PAIRHMM_RUN_FP64) ? bt->n_pairs : counters[1];
+    st.n_rescued = (mode == PAIRHMM_RUN_FP64) ? bt->n_pairs : rescued_total(counters);
     if (stats) *stats = st;
     return PAIRHMM_OK;
 }
@@ -682,7 +696,7 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
         HIP_TRY(launch_grad(true, g, all, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
     } else {
         g.scratch = ctx->grad_scratch32.ptr;
-        RescueSink rs{rescue, counters + 1, 0.0};
+        RescueSink rs{rescue, counters + 1, kFp32RescueThreshold};  // as the forward, plus the adjoint's range
         for (size_t r = runs.size(); r-- > 0;) {
             const Run &q = runs[r];
             WorkSource w{order + q.begin, q.count, nullptr, counters + 48 + r};

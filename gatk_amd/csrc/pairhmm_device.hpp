@@ -71,7 +71,13 @@ __device__ __forceinline__ float from_prev_lane(float src) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src), kDppWaveShr1, 0xF, 0xF,
                                                               true));
 }
-__device__ __forceinline__ double from_prev_lane(double src) { return wave_shr1(0.0, src); }
+__device__ __forceinline__ double from_prev_lane(double src) {
+    const uint64_t s = __builtin_bit_cast(uint64_t, src);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(s), kDppWaveShr1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(s >> 32), kDppWaveShr1, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
 
 // a += src_a(lane k-1) * mul_a and b += src_b(lane k-1) * mul_b, lane 0 reading 0 for both sources:
 // two v_fmac_f32 with a DPP wave_shr:1 source operand.  The compiler folds a DPP move into v_mul but

@@ -10,20 +10,27 @@
 // (PairHMMModel.java:169-171); priors follow LoglessPairHMM.java:84-92 with tristate correction.
 //
 // Mapping onto a CDNA wavefront (one read/haplotype pair per wave):
-//   * Rows are processed in stripes of 63: lanes 1..63 own read rows 63 s + 1 .. 63 s + 63 of stripe
-//     s and keep their row's six transition probabilities and two priors in registers; lane 0
-//     relays row 63 s, loaded from an LDS boundary row.
+//   * Rows are processed in stripes of 64, one row per lane: lane k owns row 64 s + k + 1 of stripe
+//     s and keeps its row's six transition probabilities and two priors in registers.
 //   * At step t lane k computes column c = t - k (an anti-diagonal sweep).  Its upper neighbours
 //     (row above, columns c and c-1) are lane k-1's values from the last two steps; they arrive as
-//     DPP wave_shr:1 source operands of v_mul / v_fmac (bound_ctrl: lane 0 reads 0), so the
-//     exchange costs no instructions of its own.
-//   * Lane 63 writes its cell back into the same LDS boundary row, 63 columns behind lane 0's read
-//     point, so one buffer serves all stripes.  Rows past R on the last stripe pass the last row
-//     down (M = D = 0, I = M + I of the row above), so lane 63 then writes M(R,c) + I(R,c).
+//     DPP wave_shr:1 source operands of v_mul / v_fmac, so the exchange costs no instructions of
+//     its own.
+//   * Lane 0's row above is the last row of the previous stripe, which lane 63 wrote into LDS
+//     boundary planes as it went.  Lane 0 does not read that row cell by cell: it takes two
+//     ready-made sums per column, A(c) (its diagonal term) and B(c) (its vertical term), 8 columns
+//     per LDS read ahead of use, and the DPP products leave lane 0's preloaded value in place
+//     (wave_shr:1 without bound_ctrl), so no step waits on LDS.  FP32 forms A and B in a side pass
+//     over the three planes lane 63 left (M, I, S = I + D); FP64, whose planes are twice as large,
+//     has lane 63 form them directly into two planes.
+//   * Rows past R on the last stripe pass the last row down (M = D = 0, I = M + I of the row
+//     above), so the last stripe's planes end up holding M(R,c) + I(R,c).
 //   * Cells with c <= 0 or c > H are computed too (they are exact zeros, resp. never read by a real
 //     cell), which keeps the inner loop free of masks.
 //   * The last-row sum, the unscaling and log10 are done in FP64.  FP32 launches use S = 120 and
 //     flag pairs whose scaled sum is below the underflow threshold for an FP64 recompute (S = 1020).
+//   * Launches are per LDS column class (kColClasses): a wave's LDS slice holds the haplotype and
+//     the planes for the class's longest haplotype, so short haplotypes get more waves per CU.
 #include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
 
@@ -43,8 +50,51 @@ struct WaveLds {
 template <int kCols>
 constexpr int plane_stride() { return lds_plane_stride(kCols); }
 
+// FP32 keeps three boundary planes (M, I, S); FP64 two (lane 0's sums A and B, formed by lane 63).
+template <typename T>
+constexpr int kPlanes = sizeof(T) == sizeof(double) ? 2 : 3;
+
 template <typename T, int kCols>
-constexpr size_t lds_per_wave() { return static_cast<size_t>(kCols) + 3 * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T); }
+constexpr size_t lds_per_wave() {
+    return static_cast<size_t>(kCols) + kPlanes<T> * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T);
+}
+
+// Transition probabilities and priors of the row that read position p feeds (PairHMMModel.java:
+// 158-172, LoglessPairHMM.java:84-92); positions past the read give the pass-through row
+// (MI = II = 1, everything else 0).
+template <typename T>
+struct RowTrans {
+    T MM, GM, MI, II, MD, DD, pm, px;
+    uint32_t base;
+};
+
+template <typename T>
+__device__ __forceinline__ RowTrans<T> row_trans(int p, int R, const uint8_t *__restrict__ rb,
+                                                 const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
+                                                 const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
+                                                 const DeviceTables &tab) {
+    RowTrans<T> w{0, 0, 0, 0, 0, 0, 0, 0, 0u};
+    if (p < R) {
+        w.base = rb[p];
+        const int q = rq[p], qi = ri[p], qd = rd[p], qc = rg[p];
+        const double eq = tab.qual_err[q], ei = tab.qual_err[qi], ed = tab.qual_err[qd], ec = tab.qual_err[qc];
+        const int mn = qi < qd ? qi : qd, mx = qi < qd ? qd : qi;
+        w.MM = static_cast<T>(tab.mm_prob[((mx * (mx + 1)) >> 1) + mn]);
+        w.MI = static_cast<T>(ei);
+        w.MD = static_cast<T>(ed);
+        w.GM = static_cast<T>(1.0 - ec);
+        w.II = static_cast<T>(ec);
+        w.DD = w.II;
+        w.pm = static_cast<T>(1.0 - eq);
+        w.px = static_cast<T>(eq / 3.0);
+        if (w.base == 'N') w.px = w.pm;
+    } else {
+        // Rows past R pass the last row down unchanged: M = D = 0 and I(r,c) = M(r-1,c) + I(r-1,c).
+        w.MI = 1;
+        w.II = 1;
+    }
+    return w;
+}
 
 template <typename T, int kCols>
 __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
@@ -52,7 +102,7 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
     w.hap = base;
     w.bm = reinterpret_cast<T *>(base + kCols);
     w.bi = w.bm + plane_stride<kCols>();
-    w.bs = w.bi + plane_stride<kCols>();
+    w.bs = kPlanes<T> > 2 ? w.bi + plane_stride<kCols>() : nullptr;
     return w;
 }
 
@@ -70,7 +120,7 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
 // in place (A over S, B over I).  Lane 0 reads them ahead with wave-uniform loads and brings them
 // into the DPP products, which leave lane 0 unwritten, so no step waits on LDS.
 template <typename T, int kCols, bool kHapHasN>
-__device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
+__device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
                                const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
                                const DeviceTables &tab) {
@@ -78,29 +128,9 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
     const int n_stripes = (R + kStripe - 1) / kStripe;
     const int steps = round_up(H + kWave, kUnroll);
     for (int s = 0; s < n_stripes; ++s) {
-        const int p = s * kStripe + lane;  // read position of this lane's row
-        T MM = 0, GM = 0, MI = 0, II = 0, MD = 0, DD = 0, pm = 0, px = 0;
-        uint32_t base = 0;
-        if (p < R) {
-            base = rb[p];
-            const int q = rq[p], qi = ri[p], qd = rd[p], qc = rg[p];
-            const double eq = tab.qual_err[q], ei = tab.qual_err[qi], ed = tab.qual_err[qd], ec = tab.qual_err[qc];
-            const int mn = qi < qd ? qi : qd, mx = qi < qd ? qd : qi;
-            MM = static_cast<T>(tab.mm_prob[((mx * (mx + 1)) >> 1) + mn]);
-            MI = static_cast<T>(ei);
-            MD = static_cast<T>(ed);
-            GM = static_cast<T>(1.0 - ec);
-            II = static_cast<T>(ec);
-            DD = II;
-            pm = static_cast<T>(1.0 - eq);
-            px = static_cast<T>(eq / 3.0);
-            if (base == 'N') px = pm;
-        } else {
-            // Rows past R pass the last row down unchanged: M = D = 0 and I(r,c) = M(r-1,c) + I(r-1,c),
-            // so lane 63 of the last stripe holds M(R,c) + I(R,c) in its I.
-            MI = 1;
-            II = 1;
-        }
+        const RowTrans<T> w = row_trans<T>(s * kStripe + lane, R, rb, rq, ri, rd, rg, tab);
+        const T MM = w.MM, GM = w.GM, MI = w.MI, II = w.II, MD = w.MD, DD = w.DD, pm = w.pm, px = w.px;
+        const uint32_t base = w.base;
         // Side pass: lane 0's sums A, B from the boundary row, highest columns first so that every
         // read of index j-1 precedes its overwrite.
         {
@@ -126,10 +156,9 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
         // pointer serves both, the reads 63 elements above the stores.
         constexpr int kLag = kWave - 1;
         T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *ws = L.bs + kLdsPad - kLag;
-        int lane_w = lane, lane_r = lane;
-        asm volatile("" : "+v"(lane_w), "+v"(lane_r));
+        int lane_w = lane;
+        asm volatile("" : "+v"(lane_w));
         const bool is_writer = lane_w == kLag;
-        const bool is_first = lane_r == 0;
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
             T *wmb = wm + t0, *wib = wi + t0, *wsb = ws + t0;
             T av[kUnroll], bv[kUnroll];
@@ -144,17 +173,8 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
                 const uint32_t hb = hbs[u];
                 const bool match = (hb == base) || (kHapHasN && hb == 'N');
                 const T prior = match ? pm : px;
-                T a, i;
-                if constexpr (sizeof(T) == sizeof(float)) {
-                    a = av[u];
-                    i = bv[u];
-                    dpp_upper2(a, m2, MM, s2, GM, i, m1, MI, i1, II);
-                } else {
-                    a = fma(from_prev_lane(s2), GM, from_prev_lane(m2) * MM);
-                    i = fma(from_prev_lane(i1), II, from_prev_lane(m1) * MI);
-                    a = is_first ? av[u] : a;
-                    i = is_first ? bv[u] : i;
-                }
+                T a = av[u], i = bv[u];
+                dpp_upper2(a, m2, MM, s2, GM, i, m1, MI, i1, II);
                 const T m = a * prior;
                 T d = m1 * MD;
                 d = fma(d1, DD, d);
@@ -178,6 +198,88 @@ __device__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, cons
     for (int c = 1 + lane; c <= H; c += kWave)
         acc += static_cast<double>(L.bm[kLdsPad + c]) + static_cast<double>(L.bi[kLdsPad + c]);
     return wave_sum(acc);
+}
+
+// FP64 pass.  FP64 planes are twice as large, so this pass keeps two instead of three: lane 63
+// forms lane 0's sums for the next stripe itself,
+//   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II',
+// from the next stripe's first-row transitions (MM', GM', MI', II'), which are wave-uniform.  After
+// the last stripe they are the pass-through row's (0, 0, 1, 1), so the B plane ends up holding
+// M(R,c) + I(R,c).  Lane 0 takes A and B as the old values of the DPP moves (wave_shr:1 without
+// bound_ctrl leaves lane 0 unwritten) with its own MM = MI = 1, so the step has no lane-0 select.
+// Fewer planes put more FP64 waves on a CU.
+template <int kCols, bool kHapHasN>
+__device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
+                                   const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
+                                   const uint8_t *__restrict__ rd, const uint8_t *__restrict__ rg,
+                                   const DeviceTables &tab) {
+    const int n_stripes = (R + kWave - 1) / kWave;
+    const int steps = round_up(H + kWave, kUnroll);
+    double *pa = L.bm, *pb = L.bi;  // A(c) at kLdsPad + c, B(c) at kLdsPad + c
+    using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
+    lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
+    constexpr int kLag = kWave - 1;
+    // Lane 63 computes column c = t - 63 at step t and stores A(c+1) and B(c).
+    double *wa = pa + kLdsPad - kLag + 1, *wb = pb + kLdsPad - kLag;
+    int lane_w = lane, lane_r = lane;
+    asm volatile("" : "+v"(lane_w), "+v"(lane_r));
+    const bool is_writer = lane_w == kLag;
+    const bool is_first = lane_r == 0;
+    for (int s = 0; s < n_stripes; ++s) {
+        RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
+        const RowTrans<double> nx = row_trans<double>((s + 1) * kWave, R, rb, rq, ri, rd, rg, tab);
+        if (is_first) {
+            w.MM = 1.0;
+            w.MI = 1.0;
+        }
+        double m1 = 0, i1 = 0, d1 = 0, s1 = 0;
+        double m2 = 0, s2 = 0;
+        for (int t0 = 0; t0 < steps; t0 += kUnroll) {
+            double *wab = wa + t0, *wbb = wb + t0;
+            double av[kUnroll], bv[kUnroll];
+            __builtin_memcpy(av, __builtin_assume_aligned(pa + kLdsPad + t0, 16), sizeof(av));
+            __builtin_memcpy(bv, __builtin_assume_aligned(pb + kLdsPad + t0, 16), sizeof(bv));
+            uint32_t hbs[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t hb = hbs[u];
+                const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
+                const double prior = match ? w.pm : w.px;
+                const double a = fma(from_prev_lane(s2), w.GM, wave_shr1(av[u], m2) * w.MM);
+                const double i = fma(from_prev_lane(i1), w.II, wave_shr1(bv[u], m1) * w.MI);
+                const double m = a * prior;
+                const double d = fma(d1, w.DD, m1 * w.MD);
+                const double sx = i + d;
+                if (is_writer) {
+                    wab[u] = fma(sx, nx.GM, m * nx.MM);
+                    wbb[u] = fma(i, nx.II, m * nx.MI);
+                }
+                m2 = m1;
+                s2 = s1;
+                m1 = m;
+                i1 = i;
+                d1 = d;
+                s1 = sx;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    double acc = 0.0;
+    for (int c = 1 + lane; c <= H; c += kWave) acc += pb[kLdsPad + c];
+    return wave_sum(acc);
+}
+
+template <typename T, int kCols, bool kHapHasN>
+__device__ __forceinline__ double forward_pair(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *rb,
+                                               const uint8_t *rq, const uint8_t *ri, const uint8_t *rd,
+                                               const uint8_t *rg, const DeviceTables &tab) {
+    if constexpr (sizeof(T) == sizeof(double))
+        return forward_pair_f64<kCols, kHapHasN>(L, lane, H, R, rb, rq, ri, rd, rg, tab);
+    else
+        return forward_pair_f32<T, kCols, kHapHasN>(L, lane, H, R, rb, rq, ri, rd, rg, tab);
 }
 
 template <typename T, int kCols>
@@ -210,11 +312,21 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
         L.hap[j] = v;
     }
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
-    // Row 0: M = I = 0 and D = S = 2^S / H from column 0 on; columns left of 0 are zero.
-    for (int j = lane; j < kCols; j += kWave) {
-        L.bm[j] = 0;
-        L.bi[j] = 0;
-        L.bs[j] = j >= kLdsPad ? d0 : T(0);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        // Row 0: M = I = 0 and D = S = 2^S / H from column 0 on; columns left of 0 are zero.
+        for (int j = lane; j < kCols; j += kWave) {
+            L.bm[j] = 0;
+            L.bi[j] = 0;
+            L.bs[j] = j >= kLdsPad ? d0 : T(0);
+        }
+    } else {
+        // Row 1's sums over row 0: A(c) = S(0,c-1) GM_1 = d0 GM_1 for c >= 1, B(c) = 0.
+        const T gm1 = row_trans<T>(0, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+                                   b.rd_gcp + ro, tab).GM;
+        for (int j = lane; j < kCols; j += kWave) {
+            L.bm[j] = j >= kLdsPad + 1 ? d0 * gm1 : T(0);
+            L.bi[j] = 0;
+        }
     }
     double sum;
     if (__any(has_n))
@@ -283,7 +395,8 @@ int lds_class_index(int hap_len) {
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + 3 * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))) * (fp64 ? sizeof(double) : sizeof(float));
+    return cols + (fp64 ? kPlanes<double> * sizeof(double) : kPlanes<float> * sizeof(float)) *
+                      static_cast<size_t>(lds_plane_stride(static_cast<int>(cols)));
 }
 
 namespace {

@@ -48,7 +48,7 @@ int32_t finalize(pairhmm_stream *s, Slot &sl) {
     const double *res = static_cast<const double *>(sl.hout.ptr);
     std::memcpy(sl.user_out, res, sizeof(double) * static_cast<size_t>(sl.n_pairs));
     const uint32_t *cnt = reinterpret_cast<const uint32_t *>(res + sl.n_pairs);
-    s->rescued += cnt[1];
+    s->rescued += rescued_total(cnt);
     sl.busy = false;
     sl.user_out = nullptr;
     return PAIRHMM_OK;
@@ -160,9 +160,7 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     HIP_TRY(hipStreamWaitEvent(cs, sl.h2d_done, 0), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_zero_counters(cnt, kCounterSlots, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs), PAIRHMM_ERR_KERNEL);
-    WorkSource q2{res, 0, cnt + 1, cnt + 2};
-    RescueSink none{nullptr, nullptr, 0.0};
-    HIP_TRY(launch_forward(true, v, tab, q2, none, max_h, sh64, persistent_grid(ctx, sh64, np), cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_fp64_slices(ctx, v, tab, res, slices, cnt, true, cs), PAIRHMM_ERR_KERNEL);
     double *ho = static_cast<double *>(sl.hout.ptr);
     HIP_TRY(hipMemcpyAsync(ho, v.out, 8 * np, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipMemcpyAsync(ho + np, cnt, 4 * kCounterSlots, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);

@@ -165,6 +165,22 @@ def test_precision_stress_pairs_go_to_fp64():
     assert _per_read_err(batch, got, want) <= 1e-3
 
 
+def test_low_likelihood_pairs_keep_fp32_adjoint():
+    """Pairs with log10 L between the forward's FP32 rescue threshold (L * 2^120 = 1e-28, i.e.
+    log10 L ~ -64.1) and -47 keep their adjoint in FP32; pairs below the threshold go to the FP64
+    kernel, exactly as in the forward.  Both kinds match the oracle."""
+    batch, q = _problem(31, n_regions=3, hap_len=(250, 360), read_len=(150, 220), sub=0.12)
+    want_out, want = _oracle(batch, q)
+    below = want_out + 120 * np.log10(2.0) < -28
+    window = ~below & (want_out < -47)
+    assert window.sum() >= 3 and below.sum() >= 3
+    out, got, st = _run(batch, q)
+    assert st.n_rescued == int(below.sum())
+    assert np.max(np.abs(out - want_out)) <= 1e-4
+    assert _per_read_err(batch, got, want) <= 1e-3
+    assert _rel(got["eb"], want["eb"]) <= 1e-3
+
+
 def test_autograd_function():
     batch, q = _problem(5, n_regions=2)
     w = np.random.default_rng(1).uniform(0.5, 1.5, batch.n_pairs)

@@ -204,6 +204,23 @@ def test_precision_stress_rescue(engine, engine64):
     assert np.max(np.abs(out64[sample][finite] - want[finite])) <= FP64_TOL
 
 
+def test_rescued_pairs_equal_fp64_kernel(engine):
+    """Pairs the FP32 pass hands to the FP64 kernel carry the FP64 kernel's value bit for bit (the
+    per-class rescue launches over per-slice lists); pairs it keeps are FP32 values within 1e-4 of
+    the FP64 kernel."""
+    b = synth.workload("precision_stress", n_regions=2, seed=5)
+    dev = engine.upload(b)
+    out = np.empty(b.n_pairs)
+    st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+    out64 = np.empty(b.n_pairs)
+    dev.forward(mode=_native.RUN_FP64, out=out64)
+    dev.free()
+    rescued = out + 120 * np.log10(2.0) < -28.0  # FP32 sum L * 2^120 below 1e-28
+    assert st.n_rescued == int(rescued.sum()) and 0 < st.n_rescued < b.n_pairs
+    np.testing.assert_array_equal(out[rescued], out64[rescued])
+    assert np.max(np.abs(out[~rescued] - out64[~rescued])) <= FP32_TOL
+
+
 def test_concurrent_contexts():
     """Independent contexts on two threads (Spark runs many engines per JVM,
     HaplotypeCallerSpark.java:172-183) give the same answers as one context."""
