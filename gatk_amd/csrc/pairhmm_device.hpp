@@ -121,7 +121,47 @@ __device__ __forceinline__ void dpp_upper2(float &a, float a0, float ma0, float 
         : "v"(a0), "v"(ma0), "v"(a1), "v"(ma1), "v"(b0), "v"(mb0), "v"(b1), "v"(mb1));
 }
 
-constexpr int kRows = kWave - 1;  // real rows per stripe: lane 0 relays the row above
+// a^k for 0 <= k < 128 by repeated squaring (exact to a few ulp, unlike exp2(k log2 a)).
+__device__ __forceinline__ float ipow(float a, int k) {
+    float r = 1.f, b = a;
+#pragma unroll
+    for (int bit = 0; bit < 7; ++bit) {
+        r = (k & (1 << bit)) ? r * b : r;
+        b *= b;
+    }
+    return r;
+}
+
+// Lane 63's value, wave-uniform (v_readlane).
+__device__ __forceinline__ float lane63(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
+
+// Inclusive linear-recurrence scan over the wave: y_l = sum_{k <= l} x_k a^(l-k).  Hillis-Steele
+// steps y_l += a^o y_(l-o) as v_fmac_f32 with a DPP source: row_shr 1, 2, 4, 8 within rows of 16
+// (lanes without a source add 0), then row_bcast:15 into rows 1 and 3 (times a^((l mod 16) + 1))
+// and row_bcast:31 into rows 2 and 3 (times a^(l - 31)).  s_nop 1 gives the two wait states a DPP
+// read needs after the previous VALU write of its source; the leading s_nop 4 also covers an EXEC
+// write.
+__device__ __forceinline__ float linear_scan64(float x, float a1, float a2, float a4, float a8, float p_row,
+                                               float p_31) {
+    float y = x;
+    asm("s_nop 4\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %2 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %3 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %4 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %5 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %6 row_bcast:31 row_mask:0xc bank_mask:0xf"
+        : "+&v"(y)
+        : "v"(a1), "v"(a2), "v"(a4), "v"(a8), "v"(p_row), "v"(p_31));
+    return y;
+}
 
 }  // namespace dev
 }  // namespace gatk_pairhmm

@@ -44,19 +44,19 @@ constexpr int kUnroll = 8;  // wave-steps per unrolled loop body
 template <typename T>
 struct WaveLds {
     uint8_t *hap;      // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
-    T *bm, *bi, *bs;   // [cols]: boundary row M, I and S = I + D, column c at kLdsPad + c
+    T *bm, *bi;        // [cols]: boundary planes, column c at kLdsPad + c
 };
 
 template <int kCols>
 constexpr int plane_stride() { return lds_plane_stride(kCols); }
 
-// FP32 keeps three boundary planes (M, I, S); FP64 two (lane 0's sums A and B, formed by lane 63).
-template <typename T>
-constexpr int kPlanes = sizeof(T) == sizeof(double) ? 2 : 3;
+// Two boundary planes per wave: lane 63's row (FP32: M and I, turned into lane 0's sums A and B by
+// the side pass; FP64: A and B formed by lane 63 itself).
+constexpr int kPlanes = 2;
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() {
-    return static_cast<size_t>(kCols) + kPlanes<T> * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T);
+    return static_cast<size_t>(kCols) + kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T);
 }
 
 // Transition probabilities and priors of the row that read position p feeds (PairHMMModel.java:
@@ -102,10 +102,41 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
     w.hap = base;
     w.bm = reinterpret_cast<T *>(base + kCols);
     w.bi = w.bm + plane_stride<kCols>();
-    w.bs = kPlanes<T> > 2 ? w.bi + plane_stride<kCols>() : nullptr;
     return w;
 }
 
+
+// Side pass before stripe s > 0.  Lane 63 left the previous stripe's last row r in the two planes
+// (M in P0, I in P1).  The pass rebuilds D of that row from its own recurrence,
+//   D(c) = M(c-1) MD' + D(c-1) DD'   (MD', DD' of row r),
+// and turns the row into lane 0's sums in place, in ascending column order:
+//   A(c) = M(c-1) MM0 + (I(c-1) + D(c-1)) GM0 over M,   B(c) = M(c) MI0 + I(c) II0 over I.
+// Per 64-column chunk D is a linear scan (linear_scan64) plus the carry D(j0-1) DD'^(l+1); column
+// j0-1's M, I and D come from the previous chunk's lane 63 (the chunk itself is already overwritten).
+template <int kCols>
+__device__ __forceinline__ void side_pass_f32(float *P0, float *P1, int lane, float MM0, float GM0, float MI0,
+                                              float II0, float MDr, float DDr) {
+    const float a1 = DDr, a2 = a1 * a1, a4 = a2 * a2, a8 = a4 * a4;
+    const float p_row = ipow(a1, (lane & 15) + 1);
+    const float p_31 = ipow(a1, lane >= 32 ? lane - 31 : 0);
+    const float p_in = ipow(a1, lane + 1);
+    float cm = 0.f, ci = 0.f, cd = 0.f;  // M, I, D of column j0 - 1
+    __builtin_amdgcn_wave_barrier();
+    for (int j0 = 0; j0 < kCols; j0 += kWave) {
+        const int j = j0 + lane;
+        const float mj = P0[j], ij = P1[j];
+        const float mprev = wave_shr1(cm, mj), iprev = wave_shr1(ci, ij);
+        const float dj = fmaf(cd, p_in, linear_scan64(mprev * MDr, a1, a2, a4, a8, p_row, p_31));
+        const float dprev = wave_shr1(cd, dj);
+        __builtin_amdgcn_wave_barrier();
+        P0[j] = fmaf(iprev + dprev, GM0, mprev * MM0);
+        P1[j] = fmaf(ij, II0, mj * MI0);
+        cm = lane63(mj);
+        ci = lane63(ij);
+        cd = lane63(dj);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
 
 // One pass over a pair, stripe by stripe.  Returns the scaled last-row sum (FP64, every lane).
 //
@@ -115,9 +146,10 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
 //   I(r,c) = M(r-1,c) MI + I(r-1,c) II                    [lane k-1, one step back]
 //   D(r,c) = M(r,c-1) MD + D(r,c-1) DD                    [own lane, one step back]
 // Lane 0's row above is the last row of the previous stripe, which lane 63 left in the LDS
-// boundary planes.  Before the stripe a side pass turns those planes into lane 0's two sums,
+// boundary planes (M and I; D is rebuilt by the side pass).  Before the stripe the side pass turns
+// those planes into lane 0's two sums,
 //   A(c) = M(r-1,c-1) MM + S(r-1,c-1) GM   and   B(c) = M(r-1,c) MI + I(r-1,c) II,
-// in place (A over S, B over I).  Lane 0 reads them ahead with wave-uniform loads and brings them
+// in place (A over M, B over I).  Lane 0 reads them ahead with wave-uniform loads and brings them
 // into the DPP products, which leave lane 0 unwritten, so no step waits on LDS.
 template <typename T, int kCols, bool kHapHasN>
 __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
@@ -131,22 +163,12 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
         const RowTrans<T> w = row_trans<T>(s * kStripe + lane, R, rb, rq, ri, rd, rg, tab);
         const T MM = w.MM, GM = w.GM, MI = w.MI, II = w.II, MD = w.MD, DD = w.DD, pm = w.pm, px = w.px;
         const uint32_t base = w.base;
-        // Side pass: lane 0's sums A, B from the boundary row, highest columns first so that every
-        // read of index j-1 precedes its overwrite.
-        {
-            const T MM0 = rfl(MM), GM0 = rfl(GM), MI0 = rfl(MI), II0 = rfl(II);
-            __builtin_amdgcn_wave_barrier();
-            for (int j0 = kCols - kWave; j0 >= 0; j0 -= kWave) {
-                const int j = j0 + lane;
-                const T mj = L.bm[j], ij = L.bi[j];
-                const T mp = j > 0 ? L.bm[j - 1] : T(0), sp = j > 0 ? L.bs[j - 1] : T(0);
-                __builtin_amdgcn_wave_barrier();
-                L.bs[j] = fma(sp, GM0, mp * MM0);
-                L.bi[j] = fma(ij, II0, mj * MI0);
-                __builtin_amdgcn_wave_barrier();
-            }
+        if (s > 0) {
+            // Row 64 s (read position 64 s - 1) is the previous stripe's last row; wave-uniform.
+            const RowTrans<T> up = row_trans<T>(s * kStripe - 1, R, rb, rq, ri, rd, rg, tab);
+            side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(MM), rfl(GM), rfl(MI), rfl(II), up.MD, up.DD);
         }
-        T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
+    T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
         T m2 = 0, s2 = 0;                  // own cell two steps back (column c-2)
         // Byte loads of the haplotype, eight per block.  volatile keeps them from being merged into
         // unaligned 8-byte loads; the explicit LDS address space keeps them ds_read_u8.
@@ -155,14 +177,14 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
         // Lane 63 stores column c = t - 63 at step t; lane 0 reads A, B of column c = t.  One base
         // pointer serves both, the reads 63 elements above the stores.
         constexpr int kLag = kWave - 1;
-        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *ws = L.bs + kLdsPad - kLag;
+        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag;
         int lane_w = lane;
         asm volatile("" : "+v"(lane_w));
         const bool is_writer = lane_w == kLag;
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            T *wmb = wm + t0, *wib = wi + t0, *wsb = ws + t0;
+            T *wmb = wm + t0, *wib = wi + t0;
             T av[kUnroll], bv[kUnroll];
-            __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
+            __builtin_memcpy(av, __builtin_assume_aligned(L.bm + kLdsPad + t0, 16), sizeof(av));
             __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
             uint32_t hbs[kUnroll];
 #pragma unroll
@@ -182,7 +204,6 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
                 if (is_writer) {
                     wmb[u] = m;
                     wib[u] = i;
-                    wsb[u] = sx;
                 }
                 m2 = m1;
                 s2 = s1;
@@ -312,21 +333,13 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
         L.hap[j] = v;
     }
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
-    if constexpr (sizeof(T) == sizeof(float)) {
-        // Row 0: M = I = 0 and D = S = 2^S / H from column 0 on; columns left of 0 are zero.
-        for (int j = lane; j < kCols; j += kWave) {
-            L.bm[j] = 0;
-            L.bi[j] = 0;
-            L.bs[j] = j >= kLdsPad ? d0 : T(0);
-        }
-    } else {
-        // Row 1's sums over row 0: A(c) = S(0,c-1) GM_1 = d0 GM_1 for c >= 1, B(c) = 0.
-        const T gm1 = row_trans<T>(0, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
-                                   b.rd_gcp + ro, tab).GM;
-        for (int j = lane; j < kCols; j += kWave) {
-            L.bm[j] = j >= kLdsPad + 1 ? d0 * gm1 : T(0);
-            L.bi[j] = 0;
-        }
+    // Row 1's sums over row 0 (M = I = 0, D = 2^S / H from column 0 on): A(c) = d0 GM_1 for c >= 1,
+    // B(c) = 0.
+    const T gm1 = row_trans<T>(0, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+                               b.rd_gcp + ro, tab).GM;
+    for (int j = lane; j < kCols; j += kWave) {
+        L.bm[j] = j >= kLdsPad + 1 ? d0 * gm1 : T(0);
+        L.bi[j] = 0;
     }
     double sum;
     if (__any(has_n))
@@ -395,7 +408,7 @@ int lds_class_index(int hap_len) {
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + (fp64 ? kPlanes<double> * sizeof(double) : kPlanes<float> * sizeof(float)) *
+    return cols + kPlanes * (fp64 ? sizeof(double) : sizeof(float)) *
                       static_cast<size_t>(lds_plane_stride(static_cast<int>(cols)));
 }
 
