@@ -9,15 +9,17 @@
 // against).  Outputs per pair: log10 L; per read position: w_p * d log10 L_p / d(qb, qi, qd, qc) and
 // d / d e(qb), accumulated over the pairs that share the read.
 //
-// Schedule (one pair per wave, the forward kernel's stripes and anti-diagonal skew):
-//   1. Forward over all stripes, top down, exactly as the forward kernel, saving for every stripe
-//      the relay row (the row above the stripe) and, every kTile wave-steps, every lane's state
-//      registers (a checkpoint), both in per-wave global scratch.
+// Schedule (one pair per wave, 64-row stripes with the forward kernel's anti-diagonal skew):
+//   1. Forward over all stripes, top down, with lane 0 fed its sums of the row above (A, B) by a
+//      side pass, saving in per-wave global scratch, for every stripe, lane 0's A / B planes and,
+//      every kTile wave-steps, five values per lane (M, I, D one step back; M, S two steps back)
+//      as a checkpoint.
 //   2. Adjoint over the stripes bottom up.  Wave-steps run in reverse, so with the same skew lane k
 //      finds lane k+1's adjoints from one and two steps earlier (DPP wave_shl:1); lane 63 takes the
-//      row below the stripe from an LDS row written by lane 1 of the stripe underneath.  For each
-//      tile (right to left) the kernel restores its checkpoint, recomputes kTile forward steps into
-//      registers and then runs the kTile adjoint steps over them.
+//      row below the stripe (its X and gI) from LDS planes that lane 0 of the stripe underneath
+//      wrote.  For each tile (right to left) the kernel restores its checkpoint (read one tile
+//      ahead), recomputes kTile forward steps into registers and then runs the kTile adjoint steps
+//      over them.
 //   Adjoints (X = prior * gM, S = I + D):
 //      gS(r,c) = GM_{r+1} X(r+1,c+1)
 //      gD(r,c) = gS + DD_r gD(r,c+1)

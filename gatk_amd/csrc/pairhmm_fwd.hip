@@ -198,9 +198,11 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
                 T a = av[u], i = bv[u];
                 dpp_upper2(a, m2, MM, s2, GM, i, m1, MI, i1, II);
                 const T m = a * prior;
-                T d = m1 * MD;
-                d = fma(d1, DD, d);
-                const T sx = i + d;
+                // The lane carries D' = D / MD: D'(c) = M(c-1) + D'(c-1) DD and S = I + MD D', one FMA
+                // each instead of a multiply, an FMA and an add.  D' <= sum_c M(r,c), so it keeps
+                // FP32's range even where DD is close to 1.
+                const T d = fma(d1, DD, m1);
+                const T sx = fma(MD, d, i);
                 if (is_writer) {
                     wmb[u] = m;
                     wib[u] = i;
