@@ -144,6 +144,7 @@ struct pairhmm_context {
     gatk_pairhmm::capi::DevBuf stage_dev;    // packed inputs of one computeLikelihoods call + outputs
     gatk_pairhmm::capi::HostPinned stage_host;
     gatk_pairhmm::capi::DevBuf grad_scratch32, grad_scratch64, grad_work;  // forward_backward scratch
+    gatk_pairhmm::capi::DevBuf f64_planes;   // FP64 boundary planes of computeLikelihoods calls
     int waves_per_block = 4;  // launch schedule (pairhmm_set_tuning)
     int grid_percent = 100;
     int queue_order = 0;
@@ -164,10 +165,14 @@ hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &bat
                               uint32_t *rescue_list, hipStream_t s);
 
 // Enqueues the FP64 forward kernel once per slice, each with its own slice's LDS class: over the
-// slice's rescue list (rescue_pass, the count read on the device), or over all of its queue.
+// slice's rescue list (rescue_pass, the count read on the device), or over all of its queue.  The
+// waves' boundary planes come from `planes` (plane_bytes); a launch's grid is held to what fits.
 hipError_t launch_fp64_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *list, const std::vector<QueueSlice> &slices, uint32_t *counters,
-                              bool rescue_pass, hipStream_t s);
+                              bool rescue_pass, double *planes, size_t plane_bytes, hipStream_t s);
+
+// Bytes of FP64 boundary planes that launch_fp64_slices uses at its full grids.
+size_t fp64_slices_plane_bytes(const pairhmm_context *ctx, const std::vector<QueueSlice> &slices);
 
 }  // namespace capi
 }  // namespace gatk_pairhmm

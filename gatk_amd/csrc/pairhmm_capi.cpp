@@ -29,6 +29,8 @@ struct pairhmm_batch {
     uint32_t *d_counters = nullptr; // kCounterSlots: queue heads and rescue counters
     hipGraphExec_t graph = nullptr;
     std::vector<QueueSlice> slices; // FP32 and FP64 launches over the queue, one per LDS class run
+    double *d_f64 = nullptr;        // FP64 boundary planes of the batch's FP64 launches
+    size_t f64_bytes = 0;
 };
 
 extern "C" {
@@ -78,6 +80,7 @@ int32_t pairhmm_done(pairhmm_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->stage_dev.release();
     ctx->stage_host.release();
+    ctx->f64_planes.release();
     ctx->grad_scratch32.release();
     ctx->grad_scratch64.release();
     ctx->grad_work.release();
@@ -194,15 +197,22 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     };
     WorkSource all{nullptr, static_cast<uint32_t>(n_pairs), nullptr, nullptr};
     RescueSink none{nullptr, nullptr, 0.0};
+    // FP64 boundary planes for every wave of the FP64 grid (a grid-stride launch needs them all).
+    const int grid64 = grid_for(sh64);
+    const size_t waves64 = static_cast<size_t>(grid64) * sh64.waves_per_block;
+    if (ctx->f64_planes.reserve(waves64 * fp64_plane_bytes_per_wave(max_h)) != hipSuccess)
+        return fail(PAIRHMM_ERR_OOM, "FP64 plane allocation failed");
+    b.f64_planes = static_cast<double *>(ctx->f64_planes.ptr);
+    b.f64_plane_waves = static_cast<uint32_t>(waves64);
     if (fp64_all) {
-        HIP_TRY(launch_forward(true, b, tab, all, none, max_h, sh64, grid_for(sh64), ctx->stream), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(launch_forward(true, b, tab, all, none, max_h, sh64, grid64, ctx->stream), PAIRHMM_ERR_KERNEL);
     } else {
         HIP_TRY(launch_zero_counters(counters, 4, ctx->stream), PAIRHMM_ERR_KERNEL);
         RescueSink rs{rescue, counters + 1, kFp32RescueThreshold};
         HIP_TRY(launch_forward(false, b, tab, all, rs, max_h, sh32, grid_for(sh32), ctx->stream), PAIRHMM_ERR_KERNEL);
         // FP64 recompute of the flagged pairs; the count is read on the device.
         WorkSource again{rescue, 0, counters + 1, nullptr};
-        HIP_TRY(launch_forward(true, b, tab, again, none, max_h, sh64, grid_for(sh64), ctx->stream), PAIRHMM_ERR_KERNEL);
+        HIP_TRY(launch_forward(true, b, tab, again, none, max_h, sh64, grid64, ctx->stream), PAIRHMM_ERR_KERNEL);
     }
     HIP_TRY(hipMemcpyAsync(likelihoods, b.out, sizeof(double) * n_pairs, hipMemcpyDeviceToHost, ctx->stream),
             PAIRHMM_ERR_KERNEL);
@@ -356,21 +366,38 @@ hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &bat
     return hipSuccess;
 }
 
+size_t fp64_slices_plane_bytes(const pairhmm_context *ctx, const std::vector<QueueSlice> &slices) {
+    size_t need = 0;
+    for (const QueueSlice &q : slices) {
+        const LaunchShape sh = forward_shape(true, q.max_hap_len, ctx->waves_per_block);
+        const size_t waves = static_cast<size_t>(persistent_grid(ctx, sh, q.count)) * sh.waves_per_block;
+        need = std::max(need, waves * fp64_plane_bytes_per_wave(q.max_hap_len));
+    }
+    return need;
+}
+
 hipError_t launch_fp64_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *list, const std::vector<QueueSlice> &slices, uint32_t *counters,
-                              bool rescue_pass, hipStream_t s) {
+                              bool rescue_pass, double *planes, size_t plane_bytes, hipStream_t s) {
     if (slices.size() > static_cast<size_t>(kMaxSlices)) return hipErrorInvalidValue;
     RescueSink none{nullptr, nullptr, 0.0};
     for (size_t k = 0; k < slices.size(); ++k) {
         const QueueSlice &q = slices[k];
         const LaunchShape sh = forward_shape(true, q.max_hap_len, ctx->waves_per_block);
-        if (sh.waves_per_block == 0) return hipErrorInvalidValue;
+        const size_t per_wave = fp64_plane_bytes_per_wave(q.max_hap_len);
+        if (sh.waves_per_block == 0 || per_wave == 0 || planes == nullptr) return hipErrorInvalidValue;
+        const size_t cap_waves = plane_bytes / per_wave;
+        if (cap_waves < static_cast<size_t>(sh.waves_per_block)) return hipErrorInvalidValue;
         // A rescue pass reads its item count on the device; the grid is sized for the slice's
         // capacity and surplus waves leave at their first queue fetch.
         WorkSource w = rescue_pass ? WorkSource{list + q.begin, 0, slice_rescue_count(counters, k), slice_rescue_head(counters, k)}
                                    : WorkSource{list + q.begin, static_cast<uint32_t>(q.count), nullptr, slice_head(counters, k)};
-        const hipError_t e = launch_forward(true, batch, tab, w, none, q.max_hap_len, sh,
-                                            persistent_grid(ctx, sh, q.count), s);
+        const int grid = static_cast<int>(std::min<size_t>(static_cast<size_t>(persistent_grid(ctx, sh, q.count)),
+                                                           cap_waves / sh.waves_per_block));
+        DeviceBatch v = batch;
+        v.f64_planes = planes;
+        v.f64_plane_waves = static_cast<uint32_t>(std::min<size_t>(cap_waves, 0xFFFFFFFFu));
+        const hipError_t e = launch_forward(true, v, tab, w, none, q.max_hap_len, sh, grid, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -450,6 +477,12 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     bt->d_rescue = reinterpret_cast<uint32_t *>(m + o_res);
     bt->d_counters = reinterpret_cast<uint32_t *>(m + o_cnt);
     bt->slices = std::move(slices);
+    bt->f64_bytes = fp64_slices_plane_bytes(ctx, bt->slices);
+    if (bt->f64_bytes > 0 && hipMalloc(reinterpret_cast<void **>(&bt->d_f64), bt->f64_bytes) != hipSuccess) {
+        bt->d_f64 = nullptr;
+        pairhmm_batch_free(bt);
+        return fail(PAIRHMM_ERR_OOM, "FP64 plane allocation failed");
+    }
     if (forward_shape(false, max_h, ctx->waves_per_block).waves_per_block == 0 ||
         forward_shape(true, max_h, ctx->waves_per_block).waves_per_block == 0) {
         pairhmm_batch_free(bt);
@@ -472,7 +505,8 @@ hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
     e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
-    e = launch_fp64_slices(ctx, bt->view, tab, bt->d_rescue, bt->slices, bt->d_counters, true, s);
+    e = launch_fp64_slices(ctx, bt->view, tab, bt->d_rescue, bt->slices, bt->d_counters, true, bt->d_f64, bt->f64_bytes,
+                           s);
     if (ev) (void)hipEventRecord(ev[2], s);
     return e;
 }
@@ -522,7 +556,8 @@ extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt
         DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
         HIP_TRY(launch_zero_counters(bt->d_counters, kCounterSlots, s), PAIRHMM_ERR_KERNEL);
         HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
-        HIP_TRY(launch_fp64_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, false, s),
+        HIP_TRY(launch_fp64_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, false, bt->d_f64,
+                                   bt->f64_bytes, s),
                 PAIRHMM_ERR_KERNEL);
         HIP_TRY(hipEventRecord(ev[3], s), PAIRHMM_ERR_KERNEL);
     } else {
@@ -551,6 +586,7 @@ extern "C" int32_t pairhmm_batch_free(pairhmm_batch *bt) {
     if (bt->ctx && bt->ctx->stream) (void)hipStreamSynchronize(bt->ctx->stream);
     if (bt->graph) (void)hipGraphExecDestroy(bt->graph);
     if (bt->d_mem) (void)hipFree(bt->d_mem);
+    if (bt->d_f64) (void)hipFree(bt->d_f64);
     delete bt;
     return PAIRHMM_OK;
 }

@@ -50,13 +50,14 @@ struct WaveLds {
 template <int kCols>
 constexpr int plane_stride() { return lds_plane_stride(kCols); }
 
-// Two boundary planes per wave: lane 63's row (FP32: M and I, turned into lane 0's sums A and B by
-// the side pass; FP64: A and B formed by lane 63 itself).
+// Two boundary planes per wave: lane 63's row (FP32: M and I in LDS, turned into lane 0's sums A
+// and B by the side pass; FP64: A and B formed by lane 63 itself, in global memory).
 constexpr int kPlanes = 2;
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() {
-    return static_cast<size_t>(kCols) + kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T);
+    return static_cast<size_t>(kCols) +
+           (sizeof(T) == sizeof(float) ? kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T) : 0);
 }
 
 // Transition probabilities and priors of the row that read position p feeds (PairHMMModel.java:
@@ -96,11 +97,17 @@ __device__ __forceinline__ RowTrans<T> row_trans(int p, int R, const uint8_t *__
     return w;
 }
 
+// A wave's haplotype bytes (LDS) and boundary planes: LDS for FP32, the wave's slice of the global
+// FP64 planes otherwise.
 template <typename T, int kCols>
-__device__ __forceinline__ WaveLds<T> carve(uint8_t *base) {
+__device__ __forceinline__ WaveLds<T> carve(uint8_t *base, const DeviceBatch &b, size_t gwave) {
     WaveLds<T> w;
     w.hap = base;
-    w.bm = reinterpret_cast<T *>(base + kCols);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        w.bm = reinterpret_cast<T *>(base + kCols);
+    } else {
+        w.bm = b.f64_planes + gwave * kPlanes * static_cast<size_t>(plane_stride<kCols>());
+    }
     w.bi = w.bm + plane_stride<kCols>();
     return w;
 }
@@ -223,14 +230,23 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
     return wave_sum(acc);
 }
 
-// FP64 pass.  FP64 planes are twice as large, so this pass keeps two instead of three: lane 63
-// forms lane 0's sums for the next stripe itself,
+// FP64 pass.  Lane 63 forms lane 0's sums for the next stripe itself,
 //   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II',
 // from the next stripe's first-row transitions (MM', GM', MI', II'), which are wave-uniform.  After
 // the last stripe they are the pass-through row's (0, 0, 1, 1), so the B plane ends up holding
 // M(R,c) + I(R,c).  Lane 0 takes A and B as the old values of the DPP moves (wave_shr:1 without
 // bound_ctrl leaves lane 0 unwritten) with its own MM = MI = 1, so the step has no lane-0 select.
-// Fewer planes put more FP64 waves on a CU.
+//
+// A and B live in the wave's slice of global memory rather than LDS (an FP64 plane is 8 bytes a
+// column; in LDS the long-haplotype classes held a CU to 5-7 waves); the slice stays in L2.  They
+// are interleaved, P[2 j] = A and P[2 j + 1] = B at j = kLdsPad + c, so lane 63's two stores per step
+// share one base with immediate offsets and a block's 8 columns are 16 contiguous doubles.  Lane 0's
+// values are read through a VGPR address (an opaque zero offset keeps the compiler from turning the
+// uniform load into a scalar one, whose cache does not see vector stores) and rolled in half a block
+// ahead of use: after step 3 a block's first half of the buffer is reloaded with the next block's
+// first four columns, after step 7 the second half.  A wait for the wave's stores at each stripe start
+// orders the previous stripe's stores before these loads; all accesses come from one wave on one
+// CU, so its L1 holds no stale copy.
 template <int kCols, bool kHapHasN>
 __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                    const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
@@ -238,16 +254,18 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                                    const DeviceTables &tab) {
     const int n_stripes = (R + kWave - 1) / kWave;
     const int steps = round_up(H + kWave, kUnroll);
-    double *pa = L.bm, *pb = L.bi;  // A(c) at kLdsPad + c, B(c) at kLdsPad + c
+    double *P = L.bm;  // A(c) at 2 (kLdsPad + c), B(c) at 2 (kLdsPad + c) + 1
     using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
     lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
     constexpr int kLag = kWave - 1;
-    // Lane 63 computes column c = t - 63 at step t and stores A(c+1) and B(c).
-    double *wa = pa + kLdsPad - kLag + 1, *wb = pb + kLdsPad - kLag;
-    int lane_w = lane, lane_r = lane;
+    // Lane 63 computes column c = t - 63 at step t: A(c+1) at w[2 (t - t0)], B(c) one double before.
+    double *w0 = P + 2 * (kLdsPad - kLag + 1);
+    int lane_w = lane, lane_r = lane, zoff;
     asm volatile("" : "+v"(lane_w), "+v"(lane_r));
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
     const bool is_writer = lane_w == kLag;
     const bool is_first = lane_r == 0;
+    const double *r0 = P + 2 * kLdsPad + zoff;
     for (int s = 0; s < n_stripes; ++s) {
         RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
         const RowTrans<double> nx = row_trans<double>((s + 1) * kWave, R, rb, rq, ri, rd, rg, tab);
@@ -257,11 +275,13 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
         }
         double m1 = 0, i1 = 0, d1 = 0, s1 = 0;
         double m2 = 0, s2 = 0;
+        // The previous stripe's (or the initial) stores complete before this stripe reads them.
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        double ab[2 * kUnroll];
+        __builtin_memcpy(ab, __builtin_assume_aligned(r0, 16), sizeof(ab));
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            double *wab = wa + t0, *wbb = wb + t0;
-            double av[kUnroll], bv[kUnroll];
-            __builtin_memcpy(av, __builtin_assume_aligned(pa + kLdsPad + t0, 16), sizeof(av));
-            __builtin_memcpy(bv, __builtin_assume_aligned(pb + kLdsPad + t0, 16), sizeof(bv));
+            double *wb = w0 + 2 * t0;
+            const bool next = t0 + kUnroll < steps;
             uint32_t hbs[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
@@ -271,14 +291,14 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 const uint32_t hb = hbs[u];
                 const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
                 const double prior = match ? w.pm : w.px;
-                const double a = fma(from_prev_lane(s2), w.GM, wave_shr1(av[u], m2) * w.MM);
-                const double i = fma(from_prev_lane(i1), w.II, wave_shr1(bv[u], m1) * w.MI);
+                const double a = fma(from_prev_lane(s2), w.GM, wave_shr1(ab[2 * u], m2) * w.MM);
+                const double i = fma(from_prev_lane(i1), w.II, wave_shr1(ab[2 * u + 1], m1) * w.MI);
                 const double m = a * prior;
                 const double d = fma(d1, w.DD, m1 * w.MD);
                 const double sx = i + d;
                 if (is_writer) {
-                    wab[u] = fma(sx, nx.GM, m * nx.MM);
-                    wbb[u] = fma(i, nx.II, m * nx.MI);
+                    wb[2 * u] = fma(sx, nx.GM, m * nx.MM);
+                    wb[2 * u - 1] = fma(i, nx.II, m * nx.MI);
                 }
                 m2 = m1;
                 s2 = s1;
@@ -286,12 +306,18 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 i1 = i;
                 d1 = d;
                 s1 = sx;
+                if (u == kUnroll / 2 - 1 && next)
+                    __builtin_memcpy(ab, __builtin_assume_aligned(r0 + 2 * (t0 + kUnroll), 16), sizeof(ab) / 2);
+                if (u == kUnroll - 1 && next)
+                    __builtin_memcpy(ab + kUnroll, __builtin_assume_aligned(r0 + 2 * (t0 + kUnroll) + kUnroll, 16),
+                                     sizeof(ab) / 2);
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the last stripe's B values are complete
     double acc = 0.0;
-    for (int c = 1 + lane; c <= H; c += kWave) acc += pb[kLdsPad + c];
+    for (int c = 1 + lane; c <= H; c += kWave) acc += P[2 * (kLdsPad + c) + 1];
     return wave_sum(acc);
 }
 
@@ -306,7 +332,7 @@ __device__ __forceinline__ double forward_pair(const WaveLds<T> &L, int lane, in
 }
 
 template <typename T, int kCols>
-__device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, const RescueSink &rs, uint8_t *lds,
+__device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, const RescueSink &rs, const WaveLds<T> &L,
                              int lane, uint32_t pair) {
     int r, h;
     if (b.pair_read) {
@@ -320,7 +346,6 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
     const int R = b.rd_len[r];
     const uint8_t *hap = b.hap_bases + b.hap_off[h];
     const int64_t ro = b.rd_off[r];
-    WaveLds<T> L = carve<T, kCols>(lds);
 
     // Stage the haplotype (zero padded) and the row-0 boundary: M = I = 0, D = 2^S / H
     // (LoglessPairHMM.java:30-36).
@@ -336,12 +361,18 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
     }
     const T d0 = static_cast<T>(__builtin_ldexp(1.0, Scale<T>::kExp) / static_cast<double>(H));
     // Row 1's sums over row 0 (M = I = 0, D = 2^S / H from column 0 on): A(c) = d0 GM_1 for c >= 1,
-    // B(c) = 0.
+    // B(c) = 0 (FP32: A in the M plane, B in the I plane; FP64: interleaved, A at 2 j, B at 2 j + 1).
     const T gm1 = row_trans<T>(0, R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
                                b.rd_gcp + ro, tab).GM;
     for (int j = lane; j < kCols; j += kWave) {
-        L.bm[j] = j >= kLdsPad + 1 ? d0 * gm1 : T(0);
-        L.bi[j] = 0;
+        const T a0 = j >= kLdsPad + 1 ? d0 * gm1 : T(0);
+        if constexpr (sizeof(T) == sizeof(float)) {
+            L.bm[j] = a0;
+            L.bi[j] = 0;
+        } else {
+            L.bm[2 * j] = a0;
+            L.bm[2 * j + 1] = 0;
+        }
     }
     double sum;
     if (__any(has_n))
@@ -371,7 +402,11 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
     const int wave = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
     constexpr size_t kPerWave = lds_per_wave<T, kCols>();
-    uint8_t *lds = lds_raw + kPerWave * wave;
+    const size_t gwave = static_cast<size_t>(blockIdx.x) * waves_per_block + wave;
+    // FP64 waves past the planes' capacity stay idle (queue launches only; grid-stride launches are
+    // sized to the capacity by the host).
+    if (sizeof(T) == sizeof(double) && gwave >= b.f64_plane_waves) return;
+    const WaveLds<T> L = carve<T, kCols>(lds_raw + kPerWave * wave, b, gwave);
     const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
     if (ws.head) {
         for (;;) {
@@ -379,12 +414,12 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
             // lane-divergent branch; lane 0's pre-increment value is the wave's item.
             const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
             if (it >= n_items) break;
-            process_pair<T, kCols>(b, tab, rs, lds, lane, ws.items ? ws.items[it] : it);
+            process_pair<T, kCols>(b, tab, rs, L, lane, ws.items ? ws.items[it] : it);
         }
     } else {
         const uint32_t stride = gridDim.x * waves_per_block;
-        for (uint32_t it = blockIdx.x * waves_per_block + wave; it < n_items; it += stride)
-            process_pair<T, kCols>(b, tab, rs, lds, lane, ws.items ? ws.items[it] : it);
+        for (uint32_t it = gwave; it < n_items; it += stride)
+            process_pair<T, kCols>(b, tab, rs, L, lane, ws.items ? ws.items[it] : it);
     }
 }
 
@@ -410,8 +445,12 @@ int lds_class_index(int hap_len) {
 
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + kPlanes * (fp64 ? sizeof(double) : sizeof(float)) *
-                      static_cast<size_t>(lds_plane_stride(static_cast<int>(cols)));
+    return cols + (fp64 ? 0 : kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
+}
+
+size_t fp64_plane_bytes_per_wave(int max_hap_len) {
+    const int cols = lds_columns(max_hap_len);
+    return cols == 0 ? 0 : kPlanes * sizeof(double) * static_cast<size_t>(lds_plane_stride(cols));
 }
 
 namespace {

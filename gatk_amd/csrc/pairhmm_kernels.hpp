@@ -26,6 +26,10 @@ struct DeviceBatch {
     const int32_t *pair_hap;
     int32_t n_haps;  // used only for implicit pairs
     double *out;     // log10 likelihood per pair
+    // FP64 launches: each wave's two boundary planes (2 x (kCols + 4) doubles, wave w at
+    // w * 2 * (kCols + 4)) live in global memory; waves past f64_plane_waves stay idle.
+    double *f64_planes;
+    uint32_t f64_plane_waves;
 };
 
 // Qual -> probability tables in device memory (see model_tables.hpp).
@@ -91,6 +95,9 @@ struct LaunchShape {
 
 // Largest waves-per-block <= requested whose LDS fits kMaxLdsPerBlock, and its occupancy.
 LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block);
+
+// Bytes of global boundary planes one FP64 wave needs for haplotypes up to max_hap_len (0 if too long).
+size_t fp64_plane_bytes_per_wave(int max_hap_len);
 
 hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTables &tables, const WorkSource &work,
                           const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,

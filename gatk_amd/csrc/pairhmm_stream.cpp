@@ -20,6 +20,7 @@ struct Slot {
     HostPinned hin;   // packed chunk (the device layout's input part)
     HostPinned hout;  // results + counters
     DevBuf dmem;      // device copy + outputs + rescue list + counters
+    DevBuf f64;       // FP64 boundary planes of the chunk's FP64 launches
     hipEvent_t h2d_done = nullptr;
     hipEvent_t done = nullptr;
     bool busy = false;
@@ -117,8 +118,9 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
                  o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np);
     const size_t in_bytes = off;
     const size_t o_out = take(8 * np), o_cnt = take(4 * kCounterSlots), o_res = take(4 * np);
+    const size_t f64_bytes = fp64_slices_plane_bytes(ctx, slices);
     if (sl.hin.reserve(in_bytes) != hipSuccess || sl.hout.reserve(8 * np + 4 * kCounterSlots) != hipSuccess ||
-        sl.dmem.reserve(off) != hipSuccess)
+        sl.dmem.reserve(off) != hipSuccess || sl.f64.reserve(f64_bytes) != hipSuccess)
         return fail(PAIRHMM_ERR_OOM, "stream slot allocation failed");
     uint8_t *h = static_cast<uint8_t *>(sl.hin.ptr);
     struct Cp {
@@ -160,7 +162,8 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     HIP_TRY(hipStreamWaitEvent(cs, sl.h2d_done, 0), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_zero_counters(cnt, kCounterSlots, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(launch_fp64_slices(ctx, v, tab, res, slices, cnt, true, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_fp64_slices(ctx, v, tab, res, slices, cnt, true, static_cast<double *>(sl.f64.ptr), sl.f64.cap, cs),
+            PAIRHMM_ERR_KERNEL);
     double *ho = static_cast<double *>(sl.hout.ptr);
     HIP_TRY(hipMemcpyAsync(ho, v.out, 8 * np, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipMemcpyAsync(ho + np, cnt, 4 * kCounterSlots, hipMemcpyDeviceToHost, cs), PAIRHMM_ERR_KERNEL);
@@ -202,6 +205,7 @@ extern "C" int32_t pairhmm_stream_destroy(pairhmm_stream *s) {
         sl.hin.release();
         sl.hout.release();
         sl.dmem.release();
+        sl.f64.release();
     }
     if (s->copy) (void)hipStreamDestroy(s->copy);
     delete s;
