@@ -43,6 +43,8 @@ EXPORTED_SYMBOLS = (
     "pairhmm_dragstr_analyze",
     "pairhmm_dragstr_contexts",
     "pairhmm_dragstr_impute",
+    "pairhmm_apply_pcr_error_model",
+    "pairhmm_cap_read_qualities",
 )
 
 
@@ -197,6 +199,10 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_dragstr_contexts.restype = i32
         L.pairhmm_dragstr_impute.argtypes = [vp, vp, i32, i32, vp, vp, vp, i64, i64, vp, vp]
         L.pairhmm_dragstr_impute.restype = i32
+        L.pairhmm_apply_pcr_error_model.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp]
+        L.pairhmm_apply_pcr_error_model.restype = i32
+        L.pairhmm_cap_read_qualities.argtypes = [vp, vp, vp, i64, vp, i32, i32, vp, vp, vp]
+        L.pairhmm_cap_read_qualities.restype = i32
         _lib = L
         return L
 

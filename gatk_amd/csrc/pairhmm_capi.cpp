@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -755,6 +756,55 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
     st.n_rescued = d->use_double ? d->n_pairs : host_counters[1];
     if (stats) *stats = st;
+    return PAIRHMM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Read-quality preparation
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int32_t pairhmm_apply_pcr_error_model(pairhmm_context *ctx, const uint8_t *read_bases,
+                                                 const int64_t *read_off, const int32_t *read_len, int64_t n_reads,
+                                                 int32_t pcr_model, uint8_t *ins_quals, uint8_t *del_quals) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (n_reads < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative read count");
+    if (pcr_model < PAIRHMM_PCR_NONE || pcr_model > PAIRHMM_PCR_CONSERVATIVE)
+        return fail(PAIRHMM_ERR_BAD_ARG, "unknown PCR error model");
+    if (pcr_model == PAIRHMM_PCR_NONE || n_reads == 0) return PAIRHMM_OK;  // applyPCRErrorModel returns at once
+    if (!read_bases || !read_off || !read_len || !ins_quals || !del_quals)
+        return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
+    // initializePCRErrorModel (:339-353): the quality per repeat length, in Java's double arithmetic and
+    // MathUtils.fastRound (MathUtils.java:428-430).
+    PcrQualTable table{};
+    const double rate = static_cast<double>(pcr_model);
+    for (int k = 0; k <= 20; ++k) {
+        const double x = 40.0 - std::exp(k / (rate * 3.14159265358979323846)) + 1.0;
+        const int r = x > 0.0 ? static_cast<int>(x + 0.5) : static_cast<int>(x - 0.5);
+        table.q[k] = static_cast<uint8_t>(std::max(10, r));
+    }
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_pcr_error_model(read_bases, read_off, read_len, n_reads, table, ins_quals, del_quals, ctx->stream),
+            PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
+    return PAIRHMM_OK;
+}
+
+extern "C" int32_t pairhmm_cap_read_qualities(pairhmm_context *ctx, const int64_t *read_off, const int32_t *read_len,
+                                              int64_t n_reads, const int32_t *mapping_quality,
+                                              int32_t base_quality_threshold, int32_t disable_cap_to_mapq,
+                                              uint8_t *base_quals, uint8_t *ins_quals, uint8_t *del_quals) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (n_reads < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative read count");
+    if (base_quality_threshold < 0 || base_quality_threshold > 255)
+        return fail(PAIRHMM_ERR_BAD_ARG, "base_quality_threshold must be a byte");
+    if (n_reads == 0) return PAIRHMM_OK;
+    if (!read_off || !read_len || !base_quals || !ins_quals || !del_quals || (!disable_cap_to_mapq && !mapping_quality))
+        return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_cap_qualities(read_off, read_len, n_reads, mapping_quality, base_quality_threshold,
+                                 disable_cap_to_mapq ? 1 : 0, base_quals, ins_quals, del_quals, ctx->stream),
+            PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
     return PAIRHMM_OK;
 }
 

@@ -150,4 +150,15 @@ hipError_t launch_grad_class_count(const int32_t *pair_hap, const int32_t *hap_l
 hipError_t launch_grad_class_scatter(const int32_t *pair_hap, const int32_t *hap_len, uint32_t n_pairs, int max_class,
                                      const ClassRuns &runs, uint32_t *cursors, uint32_t *order, hipStream_t stream);
 
+// Read-quality preparation (readprep.hip).  table.q[k] is the PCR-model quality for repeat length k.
+struct PcrQualTable {
+    uint8_t q[21];
+};
+hipError_t launch_pcr_error_model(const uint8_t *bases, const int64_t *read_off, const int32_t *read_len,
+                                  int64_t n_reads, const PcrQualTable &table, uint8_t *ins, uint8_t *del,
+                                  hipStream_t stream);
+hipError_t launch_cap_qualities(const int64_t *read_off, const int32_t *read_len, int64_t n_reads, const int32_t *mapq,
+                                int32_t bq_threshold, int32_t disable_mapq_cap, uint8_t *quals, uint8_t *ins,
+                                uint8_t *del, hipStream_t stream);
+
 }  // namespace gatk_pairhmm

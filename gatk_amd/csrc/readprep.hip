@@ -1,0 +1,157 @@
+// readprep.hip -- HaplotypeCaller's read-quality preparation on the device, over packed read planes:
+//   * the PCR indel error model: PairHMMLikelihoodCalculationEngine.applyPCRErrorModel
+//     (tools/walkers/haplotypecaller/PairHMMLikelihoodCalculationEngine.java:360-371) caps the
+//     insertion and deletion GOP at read positions 0 .. L-2 by a quality that falls with the tandem
+//     repeat length there (ReadLikelihoodCalculationEngine.findTandemRepeatUnits,
+//     ReadLikelihoodCalculationEngine.java:193-253, repeat units up to 8 bases, lengths capped at 20);
+//   * the minimum-quality caps of capMinimumReadQualities (:307-320): base quality capped by the
+//     read's mapping quality (unless disabled) and set to 6 below the base-quality threshold,
+//     insertion / deletion quality set to 6 below 6 -- compared as Java signed bytes, so a byte of
+//     128 or more counts as low.
+// One wave per read, lanes over its positions; every thread writes only its own position.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pairhmm_kernels.hpp"
+
+namespace gatk_pairhmm {
+namespace {
+
+constexpr int kMaxStrUnit = 8;
+constexpr int kMaxRepeat = 20;
+
+__device__ __forceinline__ bool same(const uint8_t *a, const uint8_t *b, int n) {
+    for (int k = 0; k < n; ++k)
+        if (a[k] != b[k]) return false;
+    return true;
+}
+
+// GATKVariantContextUtils.findNumberOfRepetitions (GATKVariantContextUtils.java:976-1010): how many
+// copies of unit[0, ulen) the test string [0, tlen) starts (leading) or ends (trailing) with.
+__device__ int repetitions(const uint8_t *unit, int ulen, const uint8_t *test, int tlen, bool leading) {
+    if (tlen == 0) return 0;
+    const int diff = tlen - ulen;
+    int n = 0;
+    if (leading) {
+        for (int start = 0; start <= diff; start += ulen) {
+            if (!same(test + start, unit, ulen)) return n;
+            ++n;
+        }
+    } else {
+        for (int start = diff; start >= 0; start -= ulen) {
+            if (!same(test + start, unit, ulen)) return n;
+            ++n;
+        }
+    }
+    return n;
+}
+
+// findTandemRepeatUnits(read, offset).getRight(): the repeat length at offset.
+__device__ int tandem_repeat_length(const uint8_t *read, int len, int offset) {
+    int max_bw = 0;
+    const uint8_t *bw = read + offset;  // best backward unit (initially the base itself)
+    int bw_len = 1;
+    for (int s = 1; s <= kMaxStrUnit; ++s) {
+        if (offset + 1 - s < 0) break;
+        max_bw = repetitions(read + offset - s + 1, s, read, offset + 1, false);
+        if (max_bw > 1) {
+            bw = read + offset - s + 1;
+            bw_len = s;
+            break;
+        }
+    }
+    int max_rl = max_bw;
+    if (offset < len - 1) {
+        const uint8_t *fw = read + offset + 1;
+        int fw_len = 1;
+        int max_fw = 0;
+        for (int s = 1; s <= kMaxStrUnit; ++s) {
+            if (offset + s + 1 > len) break;
+            max_fw = repetitions(read + offset + 1, s, read + offset + 1, len - offset - 1, true);
+            if (max_fw > 1) {
+                fw_len = s;
+                break;
+            }
+        }
+        if (fw_len == bw_len && same(fw, bw, fw_len)) {
+            max_rl = max_bw + max_fw;
+        } else {
+            max_bw = repetitions(fw, fw_len, read, offset + 1, false);
+            max_rl = max_fw + max_bw;
+        }
+    }
+    return max_rl > kMaxRepeat ? kMaxRepeat : max_rl;
+}
+
+__global__ __launch_bounds__(256) void pcr_error_model_kernel(const uint8_t *__restrict__ bases,
+                                                              const int64_t *__restrict__ read_off,
+                                                              const int32_t *__restrict__ read_len, int64_t n_reads,
+                                                              PcrQualTable table, uint8_t *ins, uint8_t *del) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; r < n_reads;
+         r += waves) {
+        const int64_t o = read_off[r];
+        const int len = read_len[r];
+        for (int i = 1 + lane; i < len; i += kWave) {  // position i - 1, as in the reference loop
+            const uint8_t q = table.q[tandem_repeat_length(bases + o, len, i - 1)];
+            if (ins[o + i - 1] > q) ins[o + i - 1] = q;
+            if (del[o + i - 1] > q) del[o + i - 1] = q;
+        }
+    }
+}
+
+__device__ __forceinline__ uint8_t fixed_if_low(uint8_t v, int8_t min_q) {
+    return static_cast<int8_t>(v) < min_q ? static_cast<uint8_t>(6) : v;  // QualityUtils.MIN_USABLE_Q_SCORE
+}
+
+__global__ __launch_bounds__(256) void cap_qualities_kernel(const int64_t *__restrict__ read_off,
+                                                            const int32_t *__restrict__ read_len, int64_t n_reads,
+                                                            const int32_t *__restrict__ mapq, int32_t bq_threshold,
+                                                            int32_t disable_mapq_cap, uint8_t *quals, uint8_t *ins,
+                                                            uint8_t *del) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const int8_t threshold = static_cast<int8_t>(bq_threshold);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; r < n_reads;
+         r += waves) {
+        const int64_t o = read_off[r];
+        const int len = read_len[r];
+        const int mq = disable_mapq_cap ? 255 : mapq[r];
+        for (int i = lane; i < len; i += kWave) {
+            int q = quals[o + i];
+            if (!disable_mapq_cap && mq < q) q = mq;
+            quals[o + i] = fixed_if_low(static_cast<uint8_t>(q), threshold);
+            ins[o + i] = fixed_if_low(ins[o + i], 6);
+            del[o + i] = fixed_if_low(del[o + i], 6);
+        }
+    }
+}
+
+int read_grid(int64_t n_reads) {
+    const int64_t blocks = (n_reads + 3) / 4;  // four waves (reads) per 256-thread block
+    return static_cast<int>(blocks < 1 ? 1 : (blocks > 65536 ? 65536 : blocks));
+}
+
+}  // namespace
+
+hipError_t launch_pcr_error_model(const uint8_t *bases, const int64_t *read_off, const int32_t *read_len,
+                                  int64_t n_reads, const PcrQualTable &table, uint8_t *ins, uint8_t *del,
+                                  hipStream_t stream) {
+    if (n_reads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pcr_error_model_kernel, dim3(read_grid(n_reads)), dim3(256), 0, stream, bases, read_off,
+                       read_len, n_reads, table, ins, del);
+    return hipGetLastError();
+}
+
+hipError_t launch_cap_qualities(const int64_t *read_off, const int32_t *read_len, int64_t n_reads, const int32_t *mapq,
+                                int32_t bq_threshold, int32_t disable_mapq_cap, uint8_t *quals, uint8_t *ins,
+                                uint8_t *del, hipStream_t stream) {
+    if (n_reads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cap_qualities_kernel, dim3(read_grid(n_reads)), dim3(256), 0, stream, read_off, read_len,
+                       n_reads, mapq, bq_threshold, disable_mapq_cap, quals, ins, del);
+    return hipGetLastError();
+}
+
+}  // namespace gatk_pairhmm

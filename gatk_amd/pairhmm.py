@@ -13,12 +13,17 @@ src/main/java/org/broadinstitute/hellbender/):
 * ``Implementation`` mirrors ``PairHMM.Implementation`` (utils/pairhmm/PairHMM.java:40-109) for the
   native choices; construction raises ``HardwareFeatureException`` when the device is absent, the
   signal FASTEST_AVAILABLE uses to fall back (PairHMM.java:79-98).
+* ``HipLoglessPairHMM.set_and_initialize_debug_output_stream`` writes the reference's HMM results file
+  (PairHMM.java:364-385): a header line, then one line per (read, haplotype) with the inputs and the
+  log10 likelihood in Java's ``%e`` format (``java_format_e``).
 """
 from __future__ import annotations
 
 import ctypes
 import enum
+import math
 from dataclasses import dataclass, field
+from decimal import ROUND_HALF_UP, Decimal
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -214,6 +219,43 @@ class Implementation(enum.Enum):
         return HipLoglessPairHMM(args or PairHMMNativeArguments())
 
 
+RESULTS_HEADER = "# hap-bases read-bases read-qual read-ins-qual read-del-qual gcp expected-result"
+
+
+def java_format_e(v: float) -> str:
+    """Java's String.format("%e", v) (java.util.Formatter): the double's shortest decimal digits rounded
+    HALF_UP to 7 significant digits, d.dddddde+XX with at least two exponent digits; "Infinity",
+    "-Infinity" and "NaN" for the non-finite values."""
+    if math.isnan(v):
+        return "NaN"
+    if math.isinf(v):
+        return "Infinity" if v > 0 else "-Infinity"
+    if v == 0.0:
+        return ("-" if math.copysign(1.0, v) < 0 else "") + "0.000000e+00"
+    d = Decimal(repr(float(v)))
+    exp = d.adjusted()
+    q = d.scaleb(-exp).quantize(Decimal("1.000000"), rounding=ROUND_HALF_UP)
+    if abs(q) >= 10:
+        exp += 1
+        q = d.scaleb(-exp).quantize(Decimal("1.000000"), rounding=ROUND_HALF_UP)
+    return f"{q}e{'+' if exp >= 0 else '-'}{abs(exp):02d}"
+
+
+def phred_to_fastq(quals) -> str:
+    """htsjdk SAMUtils.phredToFastq: Phred q as the character q + 33 (q in 0..93)."""
+    q = np.asarray(quals, dtype=np.int64)
+    if q.size and (q.min() < 0 or q.max() > 93):
+        raise ValueError("Phred scores must lie in 0..93 to print as FASTQ characters")
+    return bytes((q + 33).astype(np.uint8)).decode("ascii")
+
+
+def results_line(hap: bytes, read: "ReadDataHolder", lk: float) -> str:
+    """One record of the HMM results file (PairHMM.writeToResultsFileIfApplicable, PairHMM.java:377-385)."""
+    return ("\n" + bytes(_as_u8(hap)).decode("ascii") + " " + bytes(_as_u8(read.read_bases)).decode("ascii") + " " +
+            phred_to_fastq(read.read_quals) + " " + phred_to_fastq(read.insertion_gop) + " " +
+            phred_to_fastq(read.deletion_gop) + " " + phred_to_fastq(read.overall_gcp) + " " + java_format_e(lk))
+
+
 class HipLoglessPairHMM:
     """VectorLoglessPairHMM (utils/pairhmm/VectorLoglessPairHMM.java:24-169) over the gfx950 engine."""
 
@@ -225,6 +267,13 @@ class HipLoglessPairHMM:
         self._hap_index: Dict[bytes, int] = {}
         self._hap_holders: List[HaplotypeDataHolder] = []
         self._log_likelihoods: Optional[np.ndarray] = None
+        self._debug = None
+
+    def set_and_initialize_debug_output_stream(self, writer) -> None:
+        """PairHMM.setAndInitializeDebugOutputStream (PairHMM.java:364-372): attach a text writer and
+        write the results-file header; every later likelihood is recorded on it."""
+        self._debug = writer
+        writer.write(RESULTS_HEADER)
 
     def initialize(self, haplotypes: Sequence[bytes], per_sample_read_list=None, read_max_length: int = 0,
                    haplotype_max_length: int = 0) -> None:
@@ -254,6 +303,8 @@ class HipLoglessPairHMM:
             for a, hap in enumerate(log_likelihoods.alleles()):
                 idx = self._hap_index[bytes(_as_u8(hap))]
                 log_likelihoods.set(a, r, out[r * n_haps + idx])
+                if self._debug is not None:  # VectorLoglessPairHMM.java:149
+                    self._debug.write(results_line(hap, holders[r], float(out[r * n_haps + idx])))
 
     def get_log_likelihood_array(self) -> Optional[np.ndarray]:
         return self._log_likelihoods

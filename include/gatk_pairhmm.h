@@ -222,6 +222,38 @@ int32_t pairhmm_dragstr_impute(const double *gop_table, const double *gcp_table,
                                const int32_t *read_len, int64_t n_reads, int64_t read_bytes, uint8_t *out_gop,
                                uint8_t *out_gcp);
 
+/* ---- Read-quality preparation on the device ------------------------------------------------- */
+
+/* PCR indel error models (PairHMMLikelihoodCalculationEngine.PCRErrorModel,
+ * PairHMMLikelihoodCalculationEngine.java:53-61); the value is the rate factor. */
+typedef enum pairhmm_pcr_error_model {
+    PAIRHMM_PCR_NONE = 0,
+    PAIRHMM_PCR_HOSTILE = 1,
+    PAIRHMM_PCR_AGGRESSIVE = 2,
+    PAIRHMM_PCR_CONSERVATIVE = 3
+} pairhmm_pcr_error_model;
+
+/* PairHMMLikelihoodCalculationEngine.applyPCRErrorModel (PairHMMLikelihoodCalculationEngine.java:360-371)
+ * for a packed read set: at positions 0 .. L-2 of every read, the insertion and deletion GOP become
+ * min(GOP, getErrorModelAdjustedQual(repeat length, rate factor)) (:356-358), the repeat length being
+ * ReadLikelihoodCalculationEngine.findTandemRepeatUnits (ReadLikelihoodCalculationEngine.java:193-253).
+ * All array pointers are DEVICE pointers; ins_quals / del_quals are updated in place.  Synchronous on
+ * the context's stream. */
+int32_t pairhmm_apply_pcr_error_model(pairhmm_context *ctx, const uint8_t *read_bases, const int64_t *read_off,
+                                      const int32_t *read_len, int64_t n_reads, int32_t pcr_model, uint8_t *ins_quals,
+                                      uint8_t *del_quals);
+
+/* PairHMMLikelihoodCalculationEngine.capMinimumReadQualities (:307-320) for a packed read set: base
+ * quality capped by the read's mapping quality (mapping_quality[n_reads], DEVICE, ignored and may be
+ * NULL when disable_cap_to_mapq is nonzero), then set to 6 (QualityUtils.MIN_USABLE_Q_SCORE) where it
+ * is below base_quality_threshold; insertion / deletion quality set to 6 where below 6.  Comparisons
+ * are Java signed-byte comparisons, as in setToFixedValueIfTooLow.  DEVICE pointers, in place,
+ * synchronous. */
+int32_t pairhmm_cap_read_qualities(pairhmm_context *ctx, const int64_t *read_off, const int32_t *read_len,
+                                   int64_t n_reads, const int32_t *mapping_quality, int32_t base_quality_threshold,
+                                   int32_t disable_cap_to_mapq, uint8_t *base_quals, uint8_t *ins_quals,
+                                   uint8_t *del_quals);
+
 #ifdef __cplusplus
 }
 #endif
