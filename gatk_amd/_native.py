@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_dragstr_impute",
     "pairhmm_apply_pcr_error_model",
     "pairhmm_cap_read_qualities",
+    "pairhmm_dragstr_impute_device",
 )
 
 
@@ -203,6 +204,8 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_apply_pcr_error_model.restype = i32
         L.pairhmm_cap_read_qualities.argtypes = [vp, vp, vp, i64, vp, i32, i32, vp, vp, vp]
         L.pairhmm_cap_read_qualities.restype = i32
+        L.pairhmm_dragstr_impute_device.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i64, i32, vp, vp]
+        L.pairhmm_dragstr_impute_device.restype = i32
         _lib = L
         return L
 

@@ -808,6 +808,37 @@ extern "C" int32_t pairhmm_cap_read_qualities(pairhmm_context *ctx, const int64_
     return PAIRHMM_OK;
 }
 
+extern "C" int32_t pairhmm_dragstr_impute_device(pairhmm_context *ctx, const double *gop_table, const double *gcp_table,
+                                                 int32_t max_period, int32_t max_repeats, const uint8_t *read_bases,
+                                                 const int64_t *read_off, const int32_t *read_len, int64_t n_reads,
+                                                 int32_t max_read_len, uint8_t *out_gop, uint8_t *out_gcp) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (!gop_table || !gcp_table) return fail(PAIRHMM_ERR_BAD_ARG, "null table");
+    if (max_period < 1 || max_repeats < 1 || static_cast<int64_t>(max_period) * max_repeats > 512)
+        return fail(PAIRHMM_ERR_BAD_ARG, "tables must have 1..512 cells");
+    if (n_reads < 0 || max_read_len < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative sizes");
+    if (max_read_len > dragstr_max_read_length())
+        return fail(PAIRHMM_ERR_BAD_ARG, "reads longer than " + std::to_string(dragstr_max_read_length()) + " bases");
+    DragstrByteTables t{};
+    t.max_period = max_period;
+    t.max_repeats = max_repeats;
+    for (int k = 0; k < max_period * max_repeats; ++k) {
+        if (!(gop_table[k] >= 0.0 && std::isfinite(gop_table[k]) && gcp_table[k] >= 0.0 && std::isfinite(gcp_table[k])))
+            return fail(PAIRHMM_ERR_BAD_ARG, "tables must hold finite values >= 0");
+        // GOP capped at 40 (DragstrPairHMMInputScoreImputator.java:31), both rounded as Math.round.
+        t.gop[k] = static_cast<uint8_t>(static_cast<int>(std::floor(std::min(40.0, gop_table[k]) + 0.5)));
+        t.gcp[k] = static_cast<uint8_t>(static_cast<int>(std::floor(gcp_table[k] + 0.5)));
+    }
+    if (n_reads == 0) return PAIRHMM_OK;
+    if (!read_bases || !read_off || !read_len || !out_gop || !out_gcp)
+        return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_dragstr_impute(read_bases, read_off, read_len, n_reads, t, out_gop, out_gcp, ctx->stream),
+            PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
+    return PAIRHMM_OK;
+}
+
 extern "C" int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning *t) {
     if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
     if (!t) return fail(PAIRHMM_ERR_BAD_ARG, "null tuning");

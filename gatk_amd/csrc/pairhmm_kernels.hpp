@@ -161,4 +161,15 @@ hipError_t launch_cap_qualities(const int64_t *read_off, const int32_t *read_len
                                 int32_t bq_threshold, int32_t disable_mapq_cap, uint8_t *quals, uint8_t *ins,
                                 uint8_t *del, hipStream_t stream);
 
+// DRAGstr imputation on the device: per-(period, repeats) GOP / GCP bytes, already rounded and
+// capped as the imputer does, row-major [max_period][max_repeats].
+struct DragstrByteTables {
+    int32_t max_period, max_repeats;
+    uint8_t gop[512], gcp[512];
+};
+hipError_t launch_dragstr_impute(const uint8_t *bases, const int64_t *read_off, const int32_t *read_len,
+                                 int64_t n_reads, const DragstrByteTables &tables, uint8_t *out_gop, uint8_t *out_gcp,
+                                 hipStream_t stream);
+int dragstr_max_read_length();
+
 }  // namespace gatk_pairhmm

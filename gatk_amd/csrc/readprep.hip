@@ -155,3 +155,94 @@ hipError_t launch_cap_qualities(const int64_t *read_off, const int32_t *read_len
 }
 
 }  // namespace gatk_pairhmm
+
+// ---------------------------------------------------------------------------------------------
+// DRAGstr imputation on the device (DragstrPairHMMInputScoreImputator.impute,
+// DragstrPairHMMInputScoreImputator.java:53-70, over the contexts of DragstrReadSTRAnalyzer,
+// DragstrReadSTRAnalyzer.java:69-139).  The copy count of period p at position i is the reference
+// test's brute-force definition (DragstrReadSTRAnalyzerTest.java:126-171): the longest run of equal
+// adjacent p-mers through any p-mer starting in [i-p+1, i+1], one copy discounted for the p-mer at 0
+// when i < p-1; the best period has the most copies, the shorter on ties.  Each lane walks those runs
+// directly over the read staged in LDS.
+// ---------------------------------------------------------------------------------------------
+namespace gatk_pairhmm {
+namespace {
+
+constexpr int kDragstrMaxRead = 4096;  // bases of one read staged per wave
+
+__device__ __forceinline__ bool pmer_eq(const uint8_t *s, int a, int b, int p) {
+    for (int k = 0; k < p; ++k)
+        if (s[a + k] != s[b + k]) return false;
+    return true;
+}
+
+__device__ int copies_at(const uint8_t *s, int L, int i, int p) {
+    if (p > L) return 0;
+    const int lo = i - p + 1 > 0 ? i - p + 1 : 0;
+    const int hi = i + 1 < L - p ? i + 1 : L - p;
+    int best = 0;
+    for (int u = lo; u <= hi; ++u) {
+        int f = 1;  // equal p-mers from u forward
+        while (u + f * p <= L - p && pmer_eq(s, u + (f - 1) * p, u + f * p, p)) ++f;
+        int b = 0;  // and backward
+        while (u - (b + 1) * p >= 0 && pmer_eq(s, u - (b + 1) * p, u - b * p, p)) ++b;
+        int c = f + b;
+        if (u == 0 && i < p - 1) c = c - 1 > 1 ? c - 1 : 1;
+        best = c > best ? c : best;
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(256) void dragstr_impute_kernel(const uint8_t *__restrict__ bases,
+                                                             const int64_t *__restrict__ read_off,
+                                                             const int32_t *__restrict__ read_len, int64_t n_reads,
+                                                             DragstrByteTables tables, uint8_t *out_gop,
+                                                             uint8_t *out_gcp) {
+    __shared__ uint8_t stage[4][kDragstrMaxRead];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    uint8_t *s = stage[wave];
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wave; r < n_reads; r += waves) {
+        const int64_t o = read_off[r];
+        const int L = read_len[r];
+        if (L > kDragstrMaxRead) continue;  // beyond the stage (the host rejects such batches up front)
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < L; i += kWave) s[i] = bases[o + i];
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < L; i += kWave) {
+            if (i == L - 1) {  // DragstrPairHMMInputScoreImputator.java:22-23,69-70
+                out_gop[o + i] = 45;
+                out_gcp[o + i] = 10;
+                continue;
+            }
+            int best_p = 1, best_c = copies_at(s, L, i, 1);
+            for (int p = 2; p <= tables.max_period; ++p) {
+                const int c = copies_at(s, L, i, p);
+                if (c > best_c) {
+                    best_c = c;
+                    best_p = p;
+                }
+            }
+            const int rep = best_c < 1 ? 1 : (best_c > tables.max_repeats ? tables.max_repeats : best_c);
+            const int cell = (best_p - 1) * tables.max_repeats + rep - 1;
+            out_gop[o + i] = tables.gop[cell];
+            out_gcp[o + i] = tables.gcp[cell];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_dragstr_impute(const uint8_t *bases, const int64_t *read_off, const int32_t *read_len,
+                                 int64_t n_reads, const DragstrByteTables &tables, uint8_t *out_gop, uint8_t *out_gcp,
+                                 hipStream_t stream) {
+    if (n_reads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dragstr_impute_kernel, dim3(read_grid(n_reads)), dim3(256), 0, stream, bases, read_off,
+                       read_len, n_reads, tables, out_gop, out_gcp);
+    return hipGetLastError();
+}
+
+int dragstr_max_read_length() { return kDragstrMaxRead; }
+
+}  // namespace gatk_pairhmm

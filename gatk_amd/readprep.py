@@ -78,6 +78,25 @@ class ReadPrep:
                                                        int(bool(disable_cap_to_mapq)), _ptr(base_quals),
                                                        _ptr(ins_quals), _ptr(del_quals)), "cap_read_qualities")
 
+    def dragstr_impute(self, params, read_bases, read_off, read_len, out_gop, out_gcp,
+                       max_read_len: Optional[int] = None) -> None:
+        """DragstrPairHMMInputScoreImputator.impute on the device: per-position GOP / GCP bytes into
+        out_gop / out_gcp (uint8 device planes) from a ``gatk_amd.dragstr.DragstrParams``."""
+        import numpy as np
+
+        for t, dt, n in ((read_bases, torch.uint8, "read_bases"), (read_off, torch.int64, "read_off"),
+                         (read_len, torch.int32, "read_len"), (out_gop, torch.uint8, "out_gop"),
+                         (out_gcp, torch.uint8, "out_gcp")):
+            _check(t, dt, n)
+        gop = np.ascontiguousarray(params.gop_table, dtype=np.float64)
+        gcp = np.ascontiguousarray(params.gcp_table, dtype=np.float64)
+        if max_read_len is None:
+            max_read_len = int(read_len.max().item()) if read_len.numel() else 0
+        _raise_for(self.lib.pairhmm_dragstr_impute_device(
+            self.ctx, gop.ctypes.data, gcp.ctypes.data, int(gop.shape[0]), int(gop.shape[1]), _ptr(read_bases),
+            _ptr(read_off), _ptr(read_len), int(read_off.numel()), int(max_read_len), _ptr(out_gop), _ptr(out_gcp)),
+            "dragstr_impute_device")
+
     def modify_read_qualities(self, read_bases, read_off, read_len, base_quals, ins_quals, del_quals,
                               model: PCRErrorModel, mapping_quality: Optional[torch.Tensor] = None,
                               base_quality_threshold: int = DEFAULT_BASE_QUALITY_THRESHOLD,

@@ -254,6 +254,15 @@ int32_t pairhmm_cap_read_qualities(pairhmm_context *ctx, const int64_t *read_off
                                    int32_t disable_cap_to_mapq, uint8_t *base_quals, uint8_t *ins_quals,
                                    uint8_t *del_quals);
 
+/* DragstrPairHMMInputScoreImputator.impute (DragstrPairHMMInputScoreImputator.java:53-70) on the device:
+ * the same outputs as pairhmm_dragstr_impute for reads held in DEVICE memory (read_bases, read_off,
+ * read_len, out_gop, out_gcp are device pointers; the tables are host arrays, at most 512 cells).
+ * max_read_len bounds the reads (at most 4096).  Synchronous on the context's stream. */
+int32_t pairhmm_dragstr_impute_device(pairhmm_context *ctx, const double *gop_table, const double *gcp_table,
+                                      int32_t max_period, int32_t max_repeats, const uint8_t *read_bases,
+                                      const int64_t *read_off, const int32_t *read_len, int64_t n_reads,
+                                      int32_t max_read_len, uint8_t *out_gop, uint8_t *out_gcp);
+
 #ifdef __cplusplus
 }
 #endif

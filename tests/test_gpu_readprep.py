@@ -139,3 +139,32 @@ class _ConstGcp(StandardPairHMMInputScoreImputator):
         imp = super().impute(read)
         imp.gap_continuation_penalties = self.gcp
         return imp
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_dragstr_impute_device_matches_host(seed):
+    """The device DRAGstr imputer against the host imputer (itself pinned by tests/test_dragstr.py
+    against DragstrReadSTRAnalyzerTest's brute force), bit for bit, on STR-rich reads, with the
+    default tables and with a random 8 x 20 table set."""
+    from gatk_amd.dragstr import DragstrParams, DragstrPairHMMInputScoreImputator
+
+    rng = np.random.default_rng(seed)
+    reads = _reads(10 + seed)
+    bases, off, lens, *_ = _pack(reads, rng)
+
+    class _B:
+        read_bases, read_off, read_len = bases, off, lens
+
+    tables = [DragstrParams.default()]
+    g = rng.uniform(5, 50, (8, 20))
+    c = rng.uniform(2, 20, (8, 20))
+    tables.append(DragstrParams(g, c, np.zeros((8, 20)), name="random"))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    with ReadPrep() as prep:
+        for params in tables:
+            want_gop, want_gcp = DragstrPairHMMInputScoreImputator(params).impute_batch(_B)
+            d_gop = torch.zeros(len(bases), dtype=torch.uint8, device="cuda")
+            d_gcp = torch.zeros(len(bases), dtype=torch.uint8, device="cuda")
+            prep.dragstr_impute(params, t(bases), t(off), t(lens), d_gop, d_gcp)
+            np.testing.assert_array_equal(d_gop.cpu().numpy(), want_gop)
+            np.testing.assert_array_equal(d_gcp.cpu().numpy(), want_gcp)
