@@ -160,6 +160,65 @@ def streaming_leg(args):
     return d
 
 
+def readprep_leg(args):
+    """The input side on the device over the reads of the batched_uneven workload: DRAGstr GOP / GCP
+    imputation, then the PCR indel error model (CONSERVATIVE) and the minimum-quality caps.  Device
+    time per step (inputs resident); CPU baselines: the host C++ DRAGstr imputer (one thread) on all
+    reads, and the read-prep oracle (PCR + caps, pure Python) on a sample."""
+    import torch
+
+    from gatk_amd import synth
+    from gatk_amd.dragstr import DragstrPairHMMInputScoreImputator, DragstrParams
+    from gatk_amd.readprep import PCRErrorModel, ReadPrep
+    from oracle import readprep_oracle as rp
+
+    b = synth.workload("batched_uneven", n_regions=args.readprep_regions, seed=1)
+    n_bases, n_reads = len(b.read_bases), len(b.read_len)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    rng = np.random.default_rng(8)
+    mapq = rng.integers(20, 61, n_reads).astype(np.int32)
+    params = DragstrParams.default()
+    d_b, d_o, d_l, d_mq = t(b.read_bases), t(b.read_off), t(b.read_len.astype(np.int32)), t(mapq)
+    gop = torch.empty(n_bases, dtype=torch.uint8, device="cuda")
+    gcp = torch.empty(n_bases, dtype=torch.uint8, device="cuda")
+    q, i, d = t(b.read_quals.copy()), t(b.read_ins.copy()), t(b.read_del.copy())
+    with ReadPrep() as prep:
+        prep.dragstr_impute(params, d_b, d_o, d_l, gop, gcp)  # warm-up
+        prep.modify_read_qualities(d_b, d_o, d_l, q, i, d, PCRErrorModel.CONSERVATIVE, mapping_quality=d_mq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prep.dragstr_impute(params, d_b, d_o, d_l, gop, gcp)
+        t1 = time.perf_counter()
+        prep.modify_read_qualities(d_b, d_o, d_l, q, i, d, PCRErrorModel.CONSERVATIVE, mapping_quality=d_mq)
+        t2 = time.perf_counter()
+    imp = DragstrPairHMMInputScoreImputator(params)
+    c0 = time.perf_counter()
+    want_gop, want_gcp = imp.impute_batch(b)
+    c1 = time.perf_counter()
+    same = bool(np.array_equal(gop.cpu().numpy(), want_gop) and np.array_equal(gcp.cpu().numpy(), want_gcp))
+    sample = rng.choice(n_reads, size=min(args.readprep_sample, n_reads), replace=False)
+    s_bases = int(b.read_len[sample].sum())
+    c2 = time.perf_counter()
+    for r in sample:
+        o, L = int(b.read_off[r]), int(b.read_len[r])
+        wi, wd = rp.apply_pcr_error_model(bytes(b.read_bases[o:o + L]), b.read_ins[o:o + L], b.read_del[o:o + L],
+                                          "CONSERVATIVE")
+        rp.cap_minimum_read_qualities(b.read_quals[o:o + L], wi, wd, int(mapq[r]), 18, False)
+    c3 = time.perf_counter()
+    d = {"reads": n_reads, "bases": n_bases,
+         "dragstr_gbases_per_s": round(n_bases / (t1 - t0) / 1e9, 3),
+         "pcr_and_caps_gbases_per_s": round(n_bases / (t2 - t1) / 1e9, 3),
+         "dragstr_matches_host": same,
+         "cpu_dragstr_host_cpp_gbases_per_s": round(n_bases / (c1 - c0) / 1e9, 4),
+         "cpu_pcr_and_caps_oracle_gbases_per_s": round(s_bases / (c3 - c2) / 1e9, 6),
+         "cpu_sample": f"host C++ DRAGstr imputer on all reads (1 thread, {c1 - c0:.2f} s); read-prep oracle "
+                       f"(pure Python) on {len(sample)} reads ({c3 - c2:.2f} s)",
+         "workload": f"reads of batched_uneven {args.readprep_regions} regions (seed 1), default DRAGstr tables, "
+                     f"PCR model CONSERVATIVE, mapping quality 20-60, base-quality threshold 18"}
+    log(f"read prep: {d}")
+    return d
+
+
 def chunked_leg(args, chunk_regions=500):
     """The streaming workload's region mix (90% batched_uneven-shaped, 10% STR-rich precision_stress
     regions), generated chunk by chunk in this process and run through the batch API (upload +
@@ -293,6 +352,9 @@ def main() -> int:
     ap.add_argument("--chunked-regions", type=int, default=10_000,
                     help="regions of the chunked streaming-workload leg (90%% batched_uneven, 10%% STR-rich; "
                          "0 skips it)")
+    ap.add_argument("--readprep-regions", type=int, default=2_000,
+                    help="regions whose reads the input-side leg prepares (0 skips it)")
+    ap.add_argument("--readprep-sample", type=int, default=300)
     ap.add_argument("--precision-regions", type=int, default=2_000,
                     help="regions of the precision-stress leg (0 skips it)")
     ap.add_argument("--precision-sample", type=int, default=60,
@@ -402,6 +464,9 @@ def main() -> int:
     chunked = None
     if args.chunked_regions > 0:
         chunked = chunked_leg(args)
+    readprep = None
+    if args.readprep_regions > 0:
+        readprep = readprep_leg(args)
     precision = None
     if args.precision_regions > 0:
         precision = precision_leg(args, min(16, os.cpu_count() or 1))
@@ -436,6 +501,7 @@ def main() -> int:
         "streaming": stream,
         "streaming_workload_chunked": chunked,
         "precision_stress": precision,
+        "read_prep": readprep,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
