@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_apply_pcr_error_model",
     "pairhmm_cap_read_qualities",
     "pairhmm_dragstr_impute_device",
+    "pairhmm_pd_compute_likelihoods",
 )
 
 
@@ -66,6 +67,11 @@ class ReadDataHolder(ctypes.Structure):
 
 class HaplotypeDataHolder(ctypes.Structure):
     _fields_ = [("haplotype_bases", ctypes.c_void_p), ("length", ctypes.c_int32)]
+
+
+class PDHaplotypeDataHolder(ctypes.Structure):
+    _fields_ = [("haplotype_bases", ctypes.c_void_p), ("haplotype_pd_bases", ctypes.c_void_p),
+                ("length", ctypes.c_int32)]
 
 
 class NativeArguments(ctypes.Structure):
@@ -206,6 +212,9 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_cap_read_qualities.restype = i32
         L.pairhmm_dragstr_impute_device.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i64, i32, vp, vp]
         L.pairhmm_dragstr_impute_device.restype = i32
+        L.pairhmm_pd_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
+                                                     ctypes.POINTER(PDHaplotypeDataHolder), i32, vp]
+        L.pairhmm_pd_compute_likelihoods.restype = i32
         _lib = L
         return L
 

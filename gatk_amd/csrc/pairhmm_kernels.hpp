@@ -172,4 +172,32 @@ hipError_t launch_dragstr_impute(const uint8_t *bases, const int64_t *read_off, 
                                  hipStream_t stream);
 int dragstr_max_read_length();
 
+// ---- partially determined PairHMM (pdhmm.hip) ----------------------------------------------
+
+// Column code of a partially determined haplotype (built on the host from the haplotype base and
+// its PartiallyDeterminedHaplotype byte): bits 0-7 the base; bits 8-15 the bases the column also
+// accepts (the SNP alternatives A/C/G/T at the PD byte's own bit positions, or every class bit for
+// an 'N' base); kPdDelEnd when the base ends a deletion span; kPdInsideRow1 / kPdInsideLater when
+// row 1 / rows >= 2 process the column in state INSIDE_DEL.
+constexpr uint32_t kPdAltA = 8, kPdAltC = 16, kPdAltG = 32, kPdAltT = 64;
+constexpr uint32_t kPdDelEnd = 1u << 24, kPdInsideRow1 = 1u << 25, kPdInsideLater = 1u << 26;
+
+// Every read against every haplotype; out[r * n_haps + h] (read-major).
+struct PdBatch {
+    const uint32_t *hap_code;  // column codes, haplotype h at hap_off[h]
+    const int64_t *hap_off;
+    const int32_t *hap_len;
+    const uint8_t *rd_bases, *rd_quals, *rd_ins, *rd_del, *rd_gcp;
+    const int64_t *rd_off;
+    const int32_t *rd_len;
+    int32_t n_haps;
+    int32_t tristate;
+    double *out;
+};
+
+// LDS of one wave (one workgroup) for haplotypes up to max_hap_len: one 32-byte record per column.
+size_t pd_lds_bytes(int max_hap_len);
+hipError_t launch_pd_forward(const PdBatch &batch, const DeviceTables &tables, uint32_t n_pairs, int max_hap_len,
+                             int grid_blocks, hipStream_t stream);
+
 }  // namespace gatk_pairhmm

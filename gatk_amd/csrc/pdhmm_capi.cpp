@@ -1,0 +1,179 @@
+// pdhmm_capi.cpp -- C-ABI of the partially determined PairHMM: the drop-in for GKL's IntelPDHMM
+// as called by src/main/java/org/broadinstitute/hellbender/utils/pairhmm/VectorLoglessPairPDHMM.java:
+// 49-61 (load / initialize) and :72-140 (computeLikelihoods over ReadDataHolder[] and
+// HaplotypeDataHolder[] with haplotypeBases + haplotypePDBases, read-major double[] out).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace gatk_pairhmm;
+using namespace gatk_pairhmm::capi;
+
+namespace {
+
+// PartiallyDeterminedHaplotype byte layout (PartiallyDeterminedHaplotype.java:59-66).
+constexpr uint8_t kPdSnp = 1, kPdDelStart = 2, kPdDelEndByte = 4, kPdAltMask = 0x78;
+
+enum class PdState { Normal, InsideDel, AfterDel };  // LoglessPDPairHMM.java:22-31
+
+// One pass of LoglessPDPairHMM's state machine over a row (:63-136): ORs `inside_bit` into the code
+// of every column processed in INSIDE_DEL and returns the state the row ends in.
+PdState pd_state_pass(const uint8_t *pd, int H, PdState state, uint32_t inside_bit, uint32_t *code) {
+    for (int j = 0; j < H; ++j) {
+        if (state == PdState::InsideDel) code[j] |= inside_bit;
+        if (state == PdState::AfterDel) state = PdState::Normal;
+        if (pd[j] & kPdDelStart) state = PdState::InsideDel;
+        if (pd[j] & kPdDelEndByte) state = PdState::AfterDel;
+    }
+    return state;
+}
+
+// Column codes of one haplotype (layout in pairhmm_kernels.hpp).  Row 1 starts NORMAL; every later
+// row starts in the state the row before ended in (the state variable lives outside the row loop,
+// :63), which is the same state for all rows >= 2.
+void pd_column_codes(const uint8_t *bases, const uint8_t *pd, int H, uint32_t *code) {
+    for (int j = 0; j < H; ++j) {
+        const uint32_t y = bases[j];
+        uint32_t alt = (pd[j] & kPdSnp) ? (pd[j] & kPdAltMask) : 0u;
+        if (y == 'N') alt |= 0xF8u;  // an 'N' haplotype base matches every read base (:163)
+        code[j] = y | (alt << 8) | ((pd[j] & kPdDelEndByte) ? kPdDelEnd : 0u);
+    }
+    const PdState end1 = pd_state_pass(pd, H, PdState::Normal, kPdInsideRow1, code);
+    pd_state_pass(pd, H, end1, kPdInsideLater, code);
+}
+
+bool is_pd_base(uint8_t x) {
+    switch (x) {
+    case 'A': case 'a': case 'C': case 'c': case 'G': case 'g': case 'T': case 't': case 'N': return true;
+    default: return false;
+    }
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
+                                                  int32_t n_reads, const pairhmm_pd_haplotype_data_holder *haps,
+                                                  int32_t n_haps, double *likelihoods) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null (initialize was not called)");
+    if (n_reads < 0 || n_haps < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative read or haplotype count");
+    if (n_reads == 0 || n_haps == 0) return PAIRHMM_OK;
+    if (!reads || !haps || !likelihoods) return fail(PAIRHMM_ERR_BAD_ARG, "null array argument");
+    int64_t read_bytes = 0, hap_cols = 0;
+    int max_h = 1;
+    for (int32_t r = 0; r < n_reads; ++r) {
+        const pairhmm_read_data_holder &rd = reads[r];
+        int32_t st = validate_read(rd.read_bases, rd.read_quals, rd.insertion_gop, rd.deletion_gop, rd.overall_gcp,
+                                   rd.length, r);
+        if (st != PAIRHMM_OK) return st;
+        read_bytes += rd.length;
+    }
+    // isBasePDMatching throws for a read base outside ACGTacgtN on a SNP column whose haplotype base
+    // is neither that base nor 'N' (LoglessPDPairHMM.java:163, 195-196).  Haplotype side: which
+    // bases sit on SNP columns.
+    std::vector<std::array<bool, 256>> snp_bases(n_haps);
+    for (int32_t h = 0; h < n_haps; ++h) {
+        const pairhmm_pd_haplotype_data_holder &hp = haps[h];
+        if (hp.length <= 0 || hp.haplotype_bases == nullptr || hp.haplotype_pd_bases == nullptr)
+            return fail(PAIRHMM_ERR_BAD_ARG, "haplotype " + std::to_string(h) + " is empty or has a null array");
+        hap_cols += hp.length;
+        max_h = std::max(max_h, hp.length);
+        snp_bases[h].fill(false);
+        for (int j = 0; j < hp.length; ++j)
+            if (hp.haplotype_pd_bases[j] & kPdSnp) snp_bases[h][hp.haplotype_bases[j]] = true;
+    }
+    for (int32_t r = 0; r < n_reads; ++r) {
+        const pairhmm_read_data_holder &rd = reads[r];
+        for (int i = 0; i < rd.length; ++i) {
+            const uint8_t x = rd.read_bases[i];
+            if (is_pd_base(x)) continue;
+            for (int32_t h = 0; h < n_haps; ++h)
+                for (int y = 0; y < 256; ++y)
+                    if (snp_bases[h][y] && y != x && y != 'N')
+                        return fail(PAIRHMM_ERR_BAD_ARG, "read " + std::to_string(r) + " base " +
+                                                             std::to_string(static_cast<int>(x)) +
+                                                             " meets a SNP column: Found unexpected base in alt alleles");
+        }
+    }
+    const int64_t n_pairs = static_cast<int64_t>(n_reads) * n_haps;
+    if (n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "too many pairs in one call");
+    if (pd_lds_bytes(max_h) > max_lds_per_block())
+        return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
+
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t o_rb = take(read_bytes), o_rq = take(read_bytes), o_ri = take(read_bytes), o_rd = take(read_bytes),
+                 o_rg = take(read_bytes), o_hc = take(sizeof(uint32_t) * hap_cols),
+                 o_roff = take(sizeof(int64_t) * n_reads), o_rlen = take(sizeof(int32_t) * n_reads),
+                 o_hoff = take(sizeof(int64_t) * n_haps), o_hlen = take(sizeof(int32_t) * n_haps);
+    const size_t in_bytes = off;
+    const size_t o_out = take(sizeof(double) * n_pairs);
+    const size_t total = off;
+
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    if (ctx->stage_host.reserve(std::max(in_bytes, sizeof(double) * static_cast<size_t>(n_pairs))) != hipSuccess)
+        return fail(PAIRHMM_ERR_OOM, "pinned staging allocation failed");
+    if (ctx->stage_dev.reserve(total) != hipSuccess) return fail(PAIRHMM_ERR_OOM, "device staging allocation failed");
+    uint8_t *hs = static_cast<uint8_t *>(ctx->stage_host.ptr);
+    int64_t *roff = reinterpret_cast<int64_t *>(hs + o_roff);
+    int32_t *rlen = reinterpret_cast<int32_t *>(hs + o_rlen);
+    int64_t *hoff = reinterpret_cast<int64_t *>(hs + o_hoff);
+    int32_t *hlen = reinterpret_cast<int32_t *>(hs + o_hlen);
+    uint32_t *hcode = reinterpret_cast<uint32_t *>(hs + o_hc);
+    int64_t pos = 0;
+    for (int32_t r = 0; r < n_reads; ++r) {
+        const pairhmm_read_data_holder &rd = reads[r];
+        roff[r] = pos;
+        rlen[r] = rd.length;
+        if (rd.length > 0) {
+            std::memcpy(hs + o_rb + pos, rd.read_bases, rd.length);
+            std::memcpy(hs + o_rq + pos, rd.read_quals, rd.length);
+            std::memcpy(hs + o_ri + pos, rd.insertion_gop, rd.length);
+            std::memcpy(hs + o_rd + pos, rd.deletion_gop, rd.length);
+            std::memcpy(hs + o_rg + pos, rd.overall_gcp, rd.length);
+        }
+        pos += rd.length;
+    }
+    pos = 0;
+    for (int32_t h = 0; h < n_haps; ++h) {
+        hoff[h] = pos;
+        hlen[h] = haps[h].length;
+        pd_column_codes(haps[h].haplotype_bases, haps[h].haplotype_pd_bases, haps[h].length, hcode + pos);
+        pos += haps[h].length;
+    }
+    uint8_t *ds = static_cast<uint8_t *>(ctx->stage_dev.ptr);
+    HIP_TRY(hipMemcpyAsync(ds, hs, in_bytes, hipMemcpyHostToDevice, ctx->stream), PAIRHMM_ERR_KERNEL);
+
+    PdBatch b{};
+    b.hap_code = reinterpret_cast<const uint32_t *>(ds + o_hc);
+    b.hap_off = reinterpret_cast<const int64_t *>(ds + o_hoff);
+    b.hap_len = reinterpret_cast<const int32_t *>(ds + o_hlen);
+    b.rd_bases = ds + o_rb;
+    b.rd_quals = ds + o_rq;
+    b.rd_ins = ds + o_ri;
+    b.rd_del = ds + o_rd;
+    b.rd_gcp = ds + o_rg;
+    b.rd_off = reinterpret_cast<const int64_t *>(ds + o_roff);
+    b.rd_len = reinterpret_cast<const int32_t *>(ds + o_rlen);
+    b.n_haps = n_haps;
+    b.tristate = 1;
+    b.out = reinterpret_cast<double *>(ds + o_out);
+    DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
+    // One wave per workgroup and per pair; the hardware dispatcher balances uneven pairs.
+    const int grid = static_cast<int>(std::min<int64_t>(n_pairs, 1 << 20));
+    HIP_TRY(launch_pd_forward(b, tab, static_cast<uint32_t>(n_pairs), max_h, grid, ctx->stream), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipMemcpyAsync(hs, ds + o_out, sizeof(double) * n_pairs, hipMemcpyDeviceToHost, ctx->stream),
+            PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
+    std::memcpy(likelihoods, hs, sizeof(double) * n_pairs);
+    return PAIRHMM_OK;
+}

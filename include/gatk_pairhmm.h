@@ -263,6 +263,28 @@ int32_t pairhmm_dragstr_impute_device(pairhmm_context *ctx, const double *gop_ta
                                       const int64_t *read_off, const int32_t *read_len, int64_t n_reads,
                                       int32_t max_read_len, uint8_t *out_gop, uint8_t *out_gcp);
 
+/* ---- Partially determined PairHMM (drop-in for GKL's IntelPDHMM) -------------------------- */
+
+/* HaplotypeDataHolder of the PD binding (VectorLoglessPairPDHMM.java:101-106): the haplotype bases
+ * and, per base, its PartiallyDeterminedHaplotype byte (PartiallyDeterminedHaplotype.java:59-66:
+ * SNP = 1, DEL_START = 2, DEL_END = 4, alternative A/C/G/T = 8/16/32/64). */
+typedef struct pairhmm_pd_haplotype_data_holder {
+    const uint8_t *haplotype_bases;
+    const uint8_t *haplotype_pd_bases;
+    int32_t length;
+} pairhmm_pd_haplotype_data_holder;
+
+/* IntelPDHMM.computeLikelihoods(ReadDataHolder[], HaplotypeDataHolder[], double[])
+ * (VectorLoglessPairPDHMM.java:115): likelihoods[r * n_haplotypes + h] = the LoglessPDPairHMM log10
+ * likelihood (LoglessPDPairHMM.java:34-150), FP64 throughout.  IntelPDHMM.load / initialize / done
+ * (VectorLoglessPairPDHMM.java:55-60) map to pairhmm_load / pairhmm_initialize / pairhmm_done; the
+ * PDHMMNativeArguments (thread count, AVX level, OpenMP setting) have no device meaning.  A read
+ * base outside ACGTacgtN meeting a SNP column whose base is neither it nor 'N' is
+ * PAIRHMM_ERR_BAD_ARG, where LoglessPDPairHMM.isBasePDMatching throws (:195-196).  Synchronous. */
+int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
+                                       int32_t n_reads, const pairhmm_pd_haplotype_data_holder *haplotypes,
+                                       int32_t n_haplotypes, double *likelihoods);
+
 #ifdef __cplusplus
 }
 #endif

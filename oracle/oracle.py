@@ -8,7 +8,8 @@ product path never does.
   ``LoglessPairHMM`` (LoglessPairHMM.java:20-93), ``Log10PairHMM`` EXACT/ORIGINAL
   (Log10PairHMM.java:58-183), the ``PairHMMModel`` / ``QualityUtils`` / ``JacobianLogTable``
   tables (PairHMMModel.java:71-117, 373-418; QualityUtils.java:43-57; MathUtils.java:406-480),
-  plus an FP64 adjoint of the recurrence for continuous Phred parameters.
+  plus an FP64 adjoint of the recurrence for continuous Phred parameters, and the partially
+  determined ``LoglessPDPairHMM`` (LoglessPDPairHMM.java:34-194).
 * ``torch_log10_likelihood`` is an independent anti-diagonal restatement of the same continuous
   model in torch FP64, used with ``torch.autograd.gradcheck`` to pin the C adjoint.
 """
@@ -63,6 +64,12 @@ def lib() -> ctypes.CDLL:
             pair = [_u8p, ctypes.c_int, _u8p, _u8p, _u8p, _u8p, _u8p, ctypes.c_int, ctypes.c_int]
             L.oracle_logless.argtypes = pair + [_dp]
             L.oracle_logless.restype = ctypes.c_int
+            L.oracle_pd_logless.argtypes = [_u8p, _u8p, ctypes.c_int, _u8p, _u8p, _u8p, _u8p, _u8p, ctypes.c_int,
+                                            ctypes.c_int, _dp]
+            L.oracle_pd_logless.restype = ctypes.c_int
+            L.oracle_pd_logless_batch.argtypes = [ctypes.c_int32, ctypes.c_int32] + [_u8p] * 5 + [
+                _i64p, _i32p, _u8p, _u8p, _i64p, _i32p, ctypes.c_int, ctypes.c_int, _dp]
+            L.oracle_pd_logless_batch.restype = ctypes.c_int
             L.oracle_log10.argtypes = pair + [ctypes.c_int, _dp]
             L.oracle_log10.restype = ctypes.c_int
             L.oracle_logless_batch.argtypes = [ctypes.c_int64, _i32p, _i32p, _u8p, _u8p, _u8p, _u8p, _u8p,
@@ -158,6 +165,47 @@ def logless(hap, read, quals, ins, dels, gcp, tristate: bool = True) -> float:
     if st != 0:
         raise OracleError(f"oracle_logless status {st}")
     return out.value
+
+
+def pd_logless(hap, hap_pd, read, quals, ins, dels, gcp, tristate: bool = True) -> float:
+    """LoglessPDPairHMM log10 likelihood (FP64, Java arithmetic order)."""
+    (h, r, q, i, d, g), R = _pair_args(hap, read, quals, ins, dels, gcp)
+    pd = _bytes(hap_pd)
+    if len(pd) != len(h):
+        raise OracleError("hap_pd must have one byte per haplotype base")
+    out = ctypes.c_double()
+    st = lib().oracle_pd_logless(_p(h), _p(pd), len(h), _p(r), _p(q), _p(i), _p(d), _p(g), R, int(tristate),
+                                 ctypes.byref(out))
+    if st != 0:
+        raise OracleError(f"oracle_pd_logless status {st}")
+    return out.value
+
+
+def pd_logless_batch(reads, haps, n_threads: int = 1, tristate: bool = True) -> np.ndarray:
+    """Every read against every PD haplotype, read-major [n_reads, n_haps].
+
+    ``reads``: sequence of (bases, quals, ins, dels, gcp); ``haps``: sequence of (bases, pd_bytes)."""
+    rb = [tuple(_bytes(x) for x in r) for r in reads]
+    hb = [(_bytes(h), _bytes(pd)) for h, pd in haps]
+    rl = np.array([len(r[0]) for r in rb], dtype=np.int32)
+    ro = np.zeros(len(rb), dtype=np.int64)
+    if len(rb) > 1:
+        ro[1:] = np.cumsum(rl[:-1])
+    hl = np.array([len(h) for h, _ in hb], dtype=np.int32)
+    ho = np.zeros(len(hb), dtype=np.int64)
+    if len(hb) > 1:
+        ho[1:] = np.cumsum(hl[:-1])
+    planes = [np.concatenate([r[k] for r in rb]) if rb else np.zeros(0, np.uint8) for k in range(5)]
+    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in planes]
+    hcat = np.ascontiguousarray(np.concatenate([h for h, _ in hb]), dtype=np.uint8)
+    pcat = np.ascontiguousarray(np.concatenate([pd for _, pd in hb]), dtype=np.uint8)
+    out = np.empty(len(rb) * len(hb), dtype=np.float64)
+    st = lib().oracle_pd_logless_batch(len(rb), len(hb), *[_p(p) for p in planes], _p(ro, _i64p), _p(rl, _i32p),
+                                       _p(hcat), _p(pcat), _p(ho, _i64p), _p(hl, _i32p), int(tristate),
+                                       int(n_threads), _p(out, _dp))
+    if st != 0:
+        raise OracleError(f"oracle_pd_logless_batch status {st}")
+    return out.reshape(len(rb), len(hb))
 
 
 def log10_pairhmm(hap, read, quals, ins, dels, gcp, exact: bool, tristate: bool = True) -> float:

@@ -19,6 +19,15 @@
  *   - PairHMM.computeReadLikelihoodGivenHaplotypeLog10 argument checks
  *       (utils/pairhmm/PairHMM.java:284-306)
  *   - PairHMM.findFirstPositionWhereHaplotypesDiffer (utils/pairhmm/PairHMM.java:341-352)
+   - LoglessPDPairHMM (partially determined haplotypes: SNP alternatives and optional deletions)
+       (utils/pairhmm/LoglessPDPairHMM.java:13-31, 34-150, 158-194), with hapStartIndex 0 as
+       PDPairHMM.java:281-288 fixes it.  The reference's own fixture for it
+       (src/test/resources/large/expected.PDHMM.hmmresults.txt) is a git-lfs pointer in this
+       checkout, so the PD restatement is pinned by reductions instead (tests/test_pd_oracle.py):
+       no PD bits equals LoglessPairHMM bit for bit, SNP alternatives equal a substituted
+       haplotype, a deletion span reproduces the likelihood of the haplotype with the span removed,
+       and every PD cell dominates its LoglessPairHMM cell.  The I rule at a deletion end
+       (:79-84) moves results only through second-order paths; it is parity unpinned.
  * and, new (the reference has no gradient): an FP64 adjoint of the LoglessPairHMM recurrence
  * with continuous per-position Phred parameters.
  *
@@ -342,6 +351,152 @@ int oracle_logless_batch(int64_t n_pairs, const int32_t *pair_read, const int32_
         const int64_t ro = read_off[r];
         int st = oracle_logless(hap_bases + hap_off[h], hap_len[h], read_bases + ro, read_quals + ro,
                                 read_ins + ro, read_del + ro, read_gcp + ro, read_len[r], tristate, &out[p]);
+        if (st != ORACLE_OK) {
+#pragma omp atomic write
+            status = st;
+        }
+    }
+    return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Partially determined PairHMM (LoglessPDPairHMM).                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+/* PartiallyDeterminedHaplotype bit layout (PartiallyDeterminedHaplotype.java:59-66) */
+#define PD_SNP 1
+#define PD_DEL_START 2
+#define PD_DEL_END 4
+#define PD_A 8
+#define PD_C 16
+#define PD_G 32
+#define PD_T 64
+
+enum { PD_NORMAL = 0, PD_INSIDE_DEL = 1, PD_AFTER_DEL = 2 }; /* LoglessPDPairHMM.java:22-31 */
+
+/* LoglessPDPairHMM.isBasePDMatching (:175-194); -1 where the reference throws. */
+static int pd_base_matching(uint8_t x, uint8_t pd) {
+    if (!(pd & PD_SNP)) return 0;
+    switch (x) {
+    case 'A': case 'a': return (pd & PD_A) != 0;
+    case 'C': case 'c': return (pd & PD_C) != 0;
+    case 'T': case 't': return (pd & PD_T) != 0;
+    case 'G': case 'g': return (pd & PD_G) != 0;
+    default: return -1;
+    }
+}
+
+static inline double jmax(double a, double b) { return a >= b ? a : b; } /* Math.max on values >= 0 */
+
+/*
+ * LoglessPDPairHMM.subComputeReadLikelihoodGivenHaplotypeLog10 (LoglessPDPairHMM.java:34-150) with
+ * hapStartIndex = 0 (PDPairHMM.java:281-298 keeps it there).  The state machine's currentState is
+ * declared before the row loop (:63), so the state after a row's last column carries into the next
+ * row's first column.  Branch matrices (:15-17) are zero in row 0 and column 0, as the reference's
+ * never-written cells are.  Priors :158-171 with isBasePDMatching; two rolling rows per matrix.
+ */
+int oracle_pd_logless(const uint8_t *hap, const uint8_t *hap_pd, int H, const uint8_t *read, const uint8_t *quals,
+                      const uint8_t *ins, const uint8_t *del, const uint8_t *gcp, int R, int tristate, double *out) {
+    oracle_init();
+    int st = validate_pair(hap, H, read, quals, ins, del, gcp, R);
+    if (st != ORACLE_OK) return st;
+    if (hap_pd == NULL) return ORACLE_BAD_ARG;
+    const int W = H + 1;
+    double *buf = (double *)malloc(sizeof(double) * (size_t)W * 12);
+    if (!buf) return ORACLE_OOM;
+    double *Mp = buf, *Ip = buf + W, *Dp = buf + 2 * W, *BMp = buf + 3 * W, *BIp = buf + 4 * W, *BDp = buf + 5 * W;
+    double *Mc = buf + 6 * W, *Ic = buf + 7 * W, *Dc = buf + 8 * W, *BMc = buf + 9 * W, *BIc = buf + 10 * W,
+           *BDc = buf + 11 * W;
+    const double initial_value = g_initial_condition / (double)H; /* :47-51 */
+    for (int j = 0; j < W; j++) {
+        Mp[j] = Ip[j] = BMp[j] = BIp[j] = BDp[j] = 0.0;
+        Dp[j] = initial_value;
+    }
+    const double tri = tristate ? 3.0 : 1.0;
+    int state = PD_NORMAL;
+    for (int i = 1; i <= R; i++) {
+        double t[6];
+        oracle_qual_to_trans_probs((int8_t)ins[i - 1], (int8_t)del[i - 1], (int8_t)gcp[i - 1], t);
+        const uint8_t x = read[i - 1];
+        const uint8_t q = quals[i - 1];
+        const double p_match = 1.0 - g_qual_err[q];
+        const double p_mis = g_qual_err[q] / tri;
+        Mc[0] = Ic[0] = Dc[0] = BMc[0] = BIc[0] = BDc[0] = 0.0;
+        for (int j = 1; j < W; j++) {
+            const uint8_t y = hap[j - 1], pd = hap_pd[j - 1];
+            int pdm = 0;
+            if (!(x == y || x == (uint8_t)'N' || y == (uint8_t)'N')) {
+                pdm = pd_base_matching(x, pd);
+                if (pdm < 0) {
+                    free(buf);
+                    return ORACLE_BAD_ARG;
+                }
+            }
+            const double prior = (x == y || x == (uint8_t)'N' || y == (uint8_t)'N' || pdm) ? p_match : p_mis;
+            switch (state) {
+            case PD_NORMAL:
+            case PD_INSIDE_DEL:
+                if (state == PD_NORMAL) {
+                    BMc[j] = Mc[j - 1];
+                    BDc[j] = Dc[j - 1];
+                    BIc[j] = Ic[j - 1];
+                } else {
+                    BMc[j] = BMc[j - 1];
+                    BDc[j] = BDc[j - 1];
+                    BIc[j] = BIc[j - 1];
+                }
+                Mc[j] = prior * (Mp[j - 1] * t[0] + Ip[j - 1] * t[1] + Dp[j - 1] * t[1]);
+                Dc[j] = Mc[j - 1] * t[4] + Dc[j - 1] * t[5];
+                break;
+            default: /* PD_AFTER_DEL */
+                BMc[j] = jmax(BMc[j - 1], Mc[j - 1]);
+                BDc[j] = jmax(BDc[j - 1], Dc[j - 1]);
+                BIc[j] = jmax(BIc[j - 1], Ic[j - 1]);
+                Mc[j] = prior * (jmax(BMp[j - 1], Mp[j - 1]) * t[0] + jmax(BIp[j - 1], Ip[j - 1]) * t[1] +
+                                 jmax(BDp[j - 1], Dp[j - 1]) * t[1]);
+                Dc[j] = jmax(BMc[j - 1], Mc[j - 1]) * t[4] + jmax(BDc[j - 1], Dc[j - 1]) * t[5];
+                break;
+            }
+            if (pd & PD_DEL_END)
+                Ic[j] = jmax(BMp[j], Mp[j]) * t[2] + jmax(BIp[j], Ip[j]) * t[3];
+            else
+                Ic[j] = Mp[j] * t[2] + Ip[j] * t[3];
+            if (state == PD_AFTER_DEL) state = PD_NORMAL;
+            if (pd & PD_DEL_START) state = PD_INSIDE_DEL;
+            if (pd & PD_DEL_END) state = PD_AFTER_DEL;
+        }
+        double *s;
+        s = Mp; Mp = Mc; Mc = s;
+        s = Ip; Ip = Ic; Ic = s;
+        s = Dp; Dp = Dc; Dc = s;
+        s = BMp; BMp = BMc; BMc = s;
+        s = BIp; BIp = BIc; BIc = s;
+        s = BDp; BDp = BDc; BDc = s;
+    }
+    double final_sum = 0.0;
+    for (int j = 1; j < W; j++) final_sum += Mp[j] + Ip[j]; /* :132-139 */
+    *out = log10(final_sum) - g_initial_condition_log10;
+    free(buf);
+    return ORACLE_OK;
+}
+
+/* Every read against every haplotype, read-major out[r * n_haps + h] (the computeLikelihoods layout
+ * of VectorLoglessPairPDHMM.java:108-115); hap_pd is a plane beside hap_bases. */
+int oracle_pd_logless_batch(int32_t n_reads, int32_t n_haps, const uint8_t *read_bases, const uint8_t *read_quals,
+                            const uint8_t *read_ins, const uint8_t *read_del, const uint8_t *read_gcp,
+                            const int64_t *read_off, const int32_t *read_len, const uint8_t *hap_bases,
+                            const uint8_t *hap_pd, const int64_t *hap_off, const int32_t *hap_len, int tristate,
+                            int n_threads, double *out) {
+    oracle_init();
+    int status = ORACLE_OK;
+    const int64_t n_pairs = (int64_t)n_reads * n_haps;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int64_t p = 0; p < n_pairs; p++) {
+        const int r = (int)(p / n_haps), h = (int)(p % n_haps);
+        const int64_t ro = read_off[r];
+        int st = oracle_pd_logless(hap_bases + hap_off[h], hap_pd + hap_off[h], hap_len[h], read_bases + ro,
+                                   read_quals + ro, read_ins + ro, read_del + ro, read_gcp + ro, read_len[r],
+                                   tristate, &out[p]);
         if (st != ORACLE_OK) {
 #pragma omp atomic write
             status = st;
