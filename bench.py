@@ -219,6 +219,84 @@ def readprep_leg(args):
     return d
 
 
+def pd_regions(n_regions, seed=21):
+    """batched_uneven regions whose haplotypes are partially determined: per haplotype 3 SNP columns
+    (1-3 alternative bases) and one optional deletion span of 1-8 bases, PD bytes laid out as
+    PartiallyDeterminedHaplotype.getPDBytesForHaplotypes does."""
+    from gatk_amd import synth
+    from gatk_amd.pdhmm import DEL_END, DEL_START, SNP
+
+    b = synth.workload("batched_uneven", n_regions=n_regions, seed=seed)
+    rng = np.random.default_rng(seed)
+    pd = np.zeros(len(b.hap_bases), np.uint8)
+    for h in range(len(b.hap_len)):
+        o, H = int(b.hap_off[h]), int(b.hap_len[h])
+        for j in rng.choice(H, 3, replace=False):
+            pd[o + j] = SNP | (int(rng.integers(1, 16)) << 3)
+        s = int(rng.integers(0, H - 8))
+        pd[o + s] |= DEL_START
+        pd[o + s + int(rng.integers(0, 8))] |= DEL_END
+    return b, pd
+
+
+def pd_leg(args, threads):
+    """Partially determined PairHMM through its drop-in (pairhmm_pd_compute_likelihoods), one call per
+    active region as VectorLoglessPairPDHMM makes them: wall rate per call including the host->device
+    staging, and the FP64 oracle on `threads` host threads over a sample of the same regions."""
+    from gatk_amd.pairhmm import ReadDataHolder
+    from gatk_amd.pdhmm import HaplotypeDataHolder, HipPDHmm
+    from oracle import oracle
+
+    b, pd = pd_regions(args.pd_regions)
+    regions = []
+    for k in range(len(b.region_read_off) - 1):
+        r0, r1 = int(b.region_read_off[k]), int(b.region_read_off[k + 1])
+        h0, h1 = int(b.region_hap_off[k]), int(b.region_hap_off[k + 1])
+        reads = []
+        for r in range(r0, r1):
+            o, L = int(b.read_off[r]), int(b.read_len[r])
+            reads.append(tuple(np.ascontiguousarray(x[o:o + L]) for x in
+                               (b.read_bases, b.read_quals, b.read_ins, b.read_del, b.read_gcp)))
+        haps = [(bytes(b.hap_bases[int(b.hap_off[h]):int(b.hap_off[h]) + int(b.hap_len[h])]),
+                 bytes(pd[int(b.hap_off[h]):int(b.hap_off[h]) + int(b.hap_len[h])])) for h in range(h0, h1)]
+        cells = int(sum(len(r[0]) for r in reads)) * int(sum(len(h) for h, _ in haps))
+        regions.append((reads, haps, cells))
+    hmm = HipPDHmm()
+    hmm.load()
+    hmm.initialize()
+    holders = [([ReadDataHolder(*r) for r in reads],
+                [HaplotypeDataHolder(np.frombuffer(h, np.uint8), np.frombuffer(p, np.uint8)) for h, p in haps],
+                np.empty(len(reads) * len(haps))) for reads, haps, _ in regions]
+    for rh, hh, out in holders[:4]:
+        hmm.compute_likelihoods(rh, hh, out)  # warm-up
+    t0 = time.perf_counter()
+    for rh, hh, out in holders:
+        hmm.compute_likelihoods(rh, hh, out)
+    wall = time.perf_counter() - t0
+    hmm.done()
+    cells = sum(c for _, _, c in regions)
+    rng = np.random.default_rng(3)
+    sample = rng.choice(len(regions), size=min(args.pd_sample, len(regions)), replace=False)
+    s_cells, worst = 0, 0.0
+    c0 = time.perf_counter()
+    for k in sample:
+        reads, haps, c = regions[k]
+        want = oracle.pd_logless_batch(reads, haps, n_threads=threads).ravel()
+        worst = max(worst, float(np.max(np.abs(want - holders[k][2]))))
+        s_cells += c
+    c1 = time.perf_counter()
+    d = {"regions": len(regions), "pairs": int(sum(len(r) * len(h) for r, h, _ in regions)), "cells": cells,
+         "dropin_wall_gcups": round(cells / wall / 1e9, 2), "wall_s": round(wall, 3),
+         "max_abs_dlog10L_vs_oracle_sample": worst,
+         "cpu_oracle_gcups": round(s_cells / (c1 - c0) / 1e9, 4),
+         "cpu_sample": f"FP64 PD oracle on {threads} threads over {len(sample)} regions ({c1 - c0:.1f} s)",
+         "dtype": "f64",
+         "workload": f"batched_uneven {args.pd_regions} regions (seed 21), each haplotype with 3 SNP columns and "
+                     f"one 1-8 base deletion span; one computeLikelihoods call per region"}
+    log(f"pd-hmm: {d}")
+    return d
+
+
 def chunked_leg(args, chunk_regions=500):
     """The streaming workload's region mix (90% batched_uneven-shaped, 10% STR-rich precision_stress
     regions), generated chunk by chunk in this process and run through the batch API (upload +
@@ -359,6 +437,9 @@ def main() -> int:
                     help="regions of the precision-stress leg (0 skips it)")
     ap.add_argument("--precision-sample", type=int, default=60,
                     help="FP32-kept and rescued pairs each checked against the FP64 oracle")
+    ap.add_argument("--pd-regions", type=int, default=1_000,
+                    help="regions of the partially determined PairHMM leg (0 skips it)")
+    ap.add_argument("--pd-sample", type=int, default=20, help="regions the PD oracle re-scores")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -467,6 +548,9 @@ def main() -> int:
     readprep = None
     if args.readprep_regions > 0:
         readprep = readprep_leg(args)
+    pdhmm = None
+    if args.pd_regions > 0:
+        pdhmm = pd_leg(args, min(16, os.cpu_count() or 1))
     precision = None
     if args.precision_regions > 0:
         precision = precision_leg(args, min(16, os.cpu_count() or 1))
@@ -502,6 +586,7 @@ def main() -> int:
         "streaming_workload_chunked": chunked,
         "precision_stress": precision,
         "read_prep": readprep,
+        "pd_hmm": pdhmm,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
