@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_cap_read_qualities",
     "pairhmm_dragstr_impute_device",
     "pairhmm_pd_compute_likelihoods",
+    "pairhmm_normalize_and_filter_likelihoods",
 )
 
 
@@ -215,6 +216,10 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_pd_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
                                                      ctypes.POINTER(PDHaplotypeDataHolder), i32, vp]
         L.pairhmm_pd_compute_likelihoods.restype = i32
+        dbl = ctypes.c_double
+        L.pairhmm_normalize_and_filter_likelihoods.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, vp, vp, dbl,
+                                                               i32, dbl, vp, vp]
+        L.pairhmm_normalize_and_filter_likelihoods.restype = i32
         _lib = L
         return L
 

@@ -200,4 +200,25 @@ size_t pd_lds_bytes(int max_hap_len);
 hipError_t launch_pd_forward(const PdBatch &batch, const DeviceTables &tables, uint32_t n_pairs, int max_hap_len,
                              int grid_blocks, hipStream_t stream);
 
+// ---- likelihood post-processing (postprocess.hip) ---------------------------------------
+
+struct PostprocessArgs {
+    double *likelihoods;        // [n_reads * n_alleles], read-major, normalised in place
+    int64_t n_reads;
+    int32_t n_alleles;
+    int32_t ref_index;          // reference allele, -1 if none
+    int32_t normalize;          // 0: cap is -inf, no normalisation
+    int32_t symmetric;
+    double cap;
+    const uint8_t *base_quals;  // HMM base qualities (dynamic threshold only)
+    const int64_t *read_off;
+    const int32_t *read_len;
+    double max_error_per_base;
+    int32_t dynamic;
+    double scale;
+    uint8_t *keep;              // [n_reads]
+    double *threshold;          // [n_reads] or null
+};
+hipError_t launch_postprocess(const PostprocessArgs &args, hipStream_t stream);
+
 }  // namespace gatk_pairhmm

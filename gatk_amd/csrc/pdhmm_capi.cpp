@@ -1,4 +1,7 @@
-// pdhmm_capi.cpp -- C-ABI of the partially determined PairHMM: the drop-in for GKL's IntelPDHMM
+// pdhmm_capi.cpp -- C-ABI of the partially determined PairHMM (the drop-in for GKL's IntelPDHMM)
+// and of the likelihood post-processing on the device.
+//
+// PD-HMM: the drop-in for GKL's IntelPDHMM
 // as called by src/main/java/org/broadinstitute/hellbender/utils/pairhmm/VectorLoglessPairPDHMM.java:
 // 49-61 (load / initialize) and :72-140 (computeLikelihoods over ReadDataHolder[] and
 // HaplotypeDataHolder[] with haplotypeBases + haplotypePDBases, read-major double[] out).
@@ -6,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -175,5 +179,44 @@ extern "C" int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pa
             PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
     std::memcpy(likelihoods, hs, sizeof(double) * n_pairs);
+    return PAIRHMM_OK;
+}
+
+// PairHMMLikelihoodCalculationEngine.java:200-201 on the device (postprocess.hip).
+extern "C" int32_t pairhmm_normalize_and_filter_likelihoods(
+    pairhmm_context *ctx, double *likelihoods, int64_t n_reads, int32_t n_alleles, int32_t reference_allele_index,
+    double log10_mismapping_cap, int32_t symmetric, const uint8_t *hmm_base_quals, const int64_t *read_off,
+    const int32_t *read_len, double expected_error_rate_per_base, int32_t dynamic_disqualification,
+    double disqualification_scale, uint8_t *keep, double *threshold) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    if (n_reads < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative read count");
+    // AlleleLikelihoods.java:417-418 and :1352.
+    if (!(log10_mismapping_cap < 0.0)) return fail(PAIRHMM_ERR_BAD_ARG, "the minimum reference likelihood fall must be negative");
+    if (n_alleles <= 0) return fail(PAIRHMM_ERR_BAD_ARG, "unsupported for read-likelihood collections with no alleles");
+    if (reference_allele_index < -1 || reference_allele_index >= n_alleles)
+        return fail(PAIRHMM_ERR_BAD_ARG, "reference allele index out of range");
+    if (!(expected_error_rate_per_base >= 0.0)) return fail(PAIRHMM_ERR_BAD_ARG, "negative expected error rate");
+    if (n_reads == 0) return PAIRHMM_OK;
+    if (!likelihoods || !read_len || !keep || (dynamic_disqualification && (!hmm_base_quals || !read_off)))
+        return fail(PAIRHMM_ERR_BAD_ARG, "null device pointer");
+    PostprocessArgs a{};
+    a.likelihoods = likelihoods;
+    a.n_reads = n_reads;
+    a.n_alleles = n_alleles;
+    a.ref_index = reference_allele_index;
+    a.normalize = log10_mismapping_cap == -HUGE_VAL ? 0 : 1;
+    a.symmetric = symmetric ? 1 : 0;
+    a.cap = log10_mismapping_cap;
+    a.base_quals = hmm_base_quals;
+    a.read_off = read_off;
+    a.read_len = read_len;
+    a.max_error_per_base = expected_error_rate_per_base;
+    a.dynamic = dynamic_disqualification ? 1 : 0;
+    a.scale = disqualification_scale;
+    a.keep = keep;
+    a.threshold = threshold;
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_postprocess(a, ctx->stream), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
     return PAIRHMM_OK;
 }

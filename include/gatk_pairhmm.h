@@ -285,6 +285,31 @@ int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_
                                        int32_t n_reads, const pairhmm_pd_haplotype_data_holder *haplotypes,
                                        int32_t n_haplotypes, double *likelihoods);
 
+/* ---- Likelihood post-processing on the device ---------------------------------------------- */
+
+/* The two steps PairHMMLikelihoodCalculationEngine.computeReadLikelihoods applies to the finished
+ * read x haplotype matrix (PairHMMLikelihoodCalculationEngine.java:200-201), on a read-major matrix
+ * in DEVICE memory (likelihoods[r * n_alleles + a], the computeLikelihoods layout; samples simply
+ * partition the reads):
+ *   AlleleLikelihoods.normalizeLikelihoods(log10_mismapping_cap, symmetric) (AlleleLikelihoods.java:
+ *   416-458), in place; log10_mismapping_cap must be negative, -INFINITY skips the step; the
+ *   reference allele (reference_allele_index, -1 if none) is excluded from the best-allele search
+ *   unless symmetric (:505-533);
+ *   ReadLikelihoodCalculationEngine.filterPoorlyModeledEvidence(dynamic_disqualification,
+ *   expected_error_rate_per_base, disqualification_scale) (ReadLikelihoodCalculationEngine.java:
+ *   98-155): keep[r] = 0 where read r's best likelihood is below its threshold, else 1; threshold[r]
+ *   (may be NULL) receives the threshold.  hmm_base_quals / read_off / read_len describe each read's
+ *   HMM base qualities (the dynamic threshold reads them; their length is the qualified read length).
+ * DEVICE pointers; synchronous on the context's stream.  No counterpart in the native binding: the
+ * reference runs these steps in Java on the host. */
+int32_t pairhmm_normalize_and_filter_likelihoods(pairhmm_context *ctx, double *likelihoods, int64_t n_reads,
+                                                 int32_t n_alleles, int32_t reference_allele_index,
+                                                 double log10_mismapping_cap, int32_t symmetric,
+                                                 const uint8_t *hmm_base_quals, const int64_t *read_off,
+                                                 const int32_t *read_len, double expected_error_rate_per_base,
+                                                 int32_t dynamic_disqualification, double disqualification_scale,
+                                                 uint8_t *keep, double *threshold);
+
 #ifdef __cplusplus
 }
 #endif
