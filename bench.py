@@ -297,6 +297,53 @@ def pd_leg(args, threads):
     return d
 
 
+def postprocess_leg(args):
+    """Likelihood post-processing on the device over one read-major matrix resident in HBM (reads x
+    haplotypes, 150-base reads, DRAGEN dynamic threshold on): device time per call and the algorithmic
+    bytes it moves; the oracle (pure Python) on a sample of reads as the CPU baseline."""
+    import torch
+
+    from gatk_amd.postprocess import LikelihoodPostProcessor, log10_mismapping_cap
+    from oracle import postprocess_oracle as pp
+
+    n, A, L = args.post_reads, 8, 150
+    rng = np.random.default_rng(5)
+    lk = -rng.exponential(15.0, (n, A))
+    quals = rng.integers(2, 41, n * L).astype(np.uint8)
+    lens = np.full(n, L, np.int32)
+    off = (np.arange(n, dtype=np.int64) * L)
+    cap = log10_mismapping_cap(45)
+    d_lk = torch.from_numpy(lk).cuda()
+    d_q, d_len, d_off = (torch.from_numpy(x).cuda() for x in (quals, lens, off))
+    keep = torch.empty(n, dtype=torch.uint8, device="cuda")
+    th = torch.empty(n, dtype=torch.float64, device="cuda")
+    reps = 20
+    with LikelihoodPostProcessor() as p:
+        p.normalize_and_filter(d_lk, d_len, 0, cap, False, d_q, d_off, 0.02, True, 1.0, keep, th)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):  # idempotent: a second cap leaves every value in place
+            p.normalize_and_filter(d_lk, d_len, 0, cap, False, d_q, d_off, 0.02, True, 1.0, keep, th)
+        torch.cuda.synchronize()
+        per_call = (time.perf_counter() - t0) / reps
+    nbytes = n * (2 * 8 * A + L + 4 + 8 + 1 + 8)
+    sample = rng.choice(n, size=min(args.post_sample, n), replace=False)
+    got_lk, got_keep = d_lk.cpu().numpy(), keep.cpu().numpy().astype(bool)
+    c0 = time.perf_counter()
+    norm, want_keep, _ = pp.postprocess(lk[sample], [quals[off[r]:off[r] + L] for r in sample], cap, False, 0, True,
+                                        0.02, 1.0)
+    c1 = time.perf_counter()
+    same = bool(np.array_equal(norm, got_lk[sample]) and np.array_equal(want_keep, got_keep[sample]))
+    d = {"reads": n, "alleles": A, "read_len": L, "ms_per_call": round(per_call * 1e3, 3),
+         "mreads_per_s": round(n / per_call / 1e6, 1),
+         "algorithmic_gb_per_s": round(nbytes / per_call / 1e9, 1), "matches_oracle_sample": same,
+         "cpu_oracle_mreads_per_s": round(len(sample) / (c1 - c0) / 1e6, 5),
+         "cpu_sample": f"oracle (pure Python, 1 thread) on {len(sample)} reads ({c1 - c0:.2f} s)",
+         "note": "wall time per call on the context's stream (synchronous entry point), inputs in HBM"}
+    log(f"post-processing: {d}")
+    return d
+
+
 def chunked_leg(args, chunk_regions=500):
     """The streaming workload's region mix (90% batched_uneven-shaped, 10% STR-rich precision_stress
     regions), generated chunk by chunk in this process and run through the batch API (upload +
@@ -440,6 +487,9 @@ def main() -> int:
     ap.add_argument("--pd-regions", type=int, default=1_000,
                     help="regions of the partially determined PairHMM leg (0 skips it)")
     ap.add_argument("--pd-sample", type=int, default=20, help="regions the PD oracle re-scores")
+    ap.add_argument("--post-reads", type=int, default=1_000_000,
+                    help="reads of the likelihood post-processing leg (0 skips it)")
+    ap.add_argument("--post-sample", type=int, default=20_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -551,6 +601,9 @@ def main() -> int:
     pdhmm = None
     if args.pd_regions > 0:
         pdhmm = pd_leg(args, min(16, os.cpu_count() or 1))
+    post = None
+    if args.post_reads > 0:
+        post = postprocess_leg(args)
     precision = None
     if args.precision_regions > 0:
         precision = precision_leg(args, min(16, os.cpu_count() or 1))
@@ -587,6 +640,7 @@ def main() -> int:
         "precision_stress": precision,
         "read_prep": readprep,
         "pd_hmm": pdhmm,
+        "likelihood_postprocessing": post,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
