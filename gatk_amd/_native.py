@@ -48,6 +48,8 @@ EXPORTED_SYMBOLS = (
     "pairhmm_dragstr_impute_device",
     "pairhmm_pd_compute_likelihoods",
     "pairhmm_normalize_and_filter_likelihoods",
+    "pairhmm_sw_align",
+    "pairhmm_sw_align_batch",
 )
 
 
@@ -68,6 +70,11 @@ class ReadDataHolder(ctypes.Structure):
 
 class HaplotypeDataHolder(ctypes.Structure):
     _fields_ = [("haplotype_bases", ctypes.c_void_p), ("length", ctypes.c_int32)]
+
+
+class SWParameters(ctypes.Structure):
+    _fields_ = [("match_value", ctypes.c_int32), ("mismatch_penalty", ctypes.c_int32),
+                ("gap_open_penalty", ctypes.c_int32), ("gap_extend_penalty", ctypes.c_int32)]
 
 
 class PDHaplotypeDataHolder(ctypes.Structure):
@@ -220,6 +227,12 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_normalize_and_filter_likelihoods.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, vp, vp, dbl,
                                                                i32, dbl, vp, vp]
         L.pairhmm_normalize_and_filter_likelihoods.restype = i32
+        L.pairhmm_sw_align.argtypes = [vp, vp, i32, vp, i32, ctypes.POINTER(SWParameters), i32, ctypes.c_char_p,
+                                       i32, ctypes.POINTER(i32)]
+        L.pairhmm_sw_align.restype = i32
+        L.pairhmm_sw_align_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32, vp,
+                                             vp, vp, vp]
+        L.pairhmm_sw_align_batch.restype = i32
         _lib = L
         return L
 

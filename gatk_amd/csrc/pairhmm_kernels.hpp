@@ -221,4 +221,28 @@ struct PostprocessArgs {
 };
 hipError_t launch_postprocess(const PostprocessArgs &args, hipStream_t stream);
 
+// ---- Smith-Waterman (sw.hip) ----------------------------------------------------------------
+
+struct SwBatch {
+    const uint8_t *ref_bases;
+    const int64_t *ref_off;
+    const int32_t *ref_len;
+    const uint8_t *alt_bases;
+    const int64_t *alt_off;
+    const int32_t *alt_len;
+    int32_t match, mismatch, open, extend;  // SWParameters
+    int32_t strategy;                       // 0 SOFTCLIP, 1 INDEL, 2 LEADING_INDEL, 3 IGNORE
+    int32_t max_ref_len;
+    uint8_t *scratch;                       // per wave: backtrack codes, last column, bottom row
+    size_t scratch_per_wave, bt_bytes_per_wave;
+    uint32_t *elems;                        // CIGAR elements (len << 2 | op), reference list order
+    const int64_t *elem_off;                // per pair; capacity ref_len + alt_len + 4
+    int32_t *n_elems;
+    int32_t *offset;                        // alignment offset
+};
+size_t sw_lds_bytes(int max_ref_len);
+size_t sw_bt_bytes_per_wave(int max_ref_len, int max_alt_len);
+size_t sw_scratch_bytes_per_wave(int max_ref_len, int max_alt_len);
+hipError_t launch_sw(const SwBatch &batch, uint32_t n_pairs, int grid_blocks, hipStream_t stream);
+
 }  // namespace gatk_pairhmm

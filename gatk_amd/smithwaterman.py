@@ -1,0 +1,150 @@
+"""Smith-Waterman on the gfx950 engine: the drop-in for GKL's SWAlignerNativeBinding.
+
+``HipSmithWatermanBinding`` mirrors the binding (load / align / close) over ``pairhmm_sw_align``;
+``SWNativeAlignerWrapper`` mirrors the reference's wrapper around any binding
+(src/main/java/org/broadinstitute/hellbender/utils/smithwaterman/SWNativeAlignerWrapper.java:19-60),
+including its exact-substring shortcut; ``align_batch`` scores many pairs in one launch
+(``pairhmm_sw_align_batch``).  The alignment itself always runs in sw.hip.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native
+from .pairhmm import _as_u8, _raise_for
+
+
+@dataclass(frozen=True)
+class SWParameters:
+    match_value: int
+    mismatch_penalty: int
+    gap_open_penalty: int
+    gap_extend_penalty: int
+
+    def native(self) -> _native.SWParameters:
+        return _native.SWParameters(self.match_value, self.mismatch_penalty, self.gap_open_penalty,
+                                    self.gap_extend_penalty)
+
+
+# SmithWatermanAlignmentConstants.java:30-68
+ORIGINAL_DEFAULT = SWParameters(3, -1, -4, -3)
+STANDARD_NGS = SWParameters(25, -50, -110, -6)
+NEW_SW_PARAMETERS = SWParameters(200, -150, -260, -11)
+ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS = SWParameters(10, -15, -30, -5)
+
+
+class SWOverhangStrategy(enum.IntEnum):
+    SOFTCLIP = 0
+    INDEL = 1
+    LEADING_INDEL = 2
+    IGNORE = 3
+
+
+@dataclass(frozen=True)
+class SmithWatermanAlignment:
+    cigar: str
+    alignment_offset: int
+
+
+def last_index_of(reference: bytes, query: bytes) -> int:
+    """Utils.lastIndexOf (Utils.java:1124-1138): the last start of query in reference, or -1."""
+    return bytes(reference).rfind(bytes(query))
+
+
+def _cigar_cap(ref_len: int, alt_len: int) -> int:
+    return 12 * (ref_len + alt_len + 4) + 1
+
+
+class HipSmithWatermanBinding:
+    """SWAlignerNativeBinding over the gfx950 engine."""
+
+    def __init__(self) -> None:
+        self._lib = None
+        self._ctx = ctypes.c_void_p()
+
+    def load(self, tmp_dir=None) -> bool:
+        self._lib = _native.load_library()
+        if not self._lib.pairhmm_load():
+            return False
+        a = _native.NativeArguments(1, 0)
+        _raise_for(self._lib.pairhmm_initialize(ctypes.byref(a), ctypes.byref(self._ctx)), "initialize")
+        return True
+
+    def align(self, ref, alt, parameters: SWParameters, strategy: SWOverhangStrategy) -> Tuple[str, int]:
+        r, a = np.ascontiguousarray(_as_u8(ref)), np.ascontiguousarray(_as_u8(alt))
+        cap = _cigar_cap(len(r), len(a))
+        buf = ctypes.create_string_buffer(cap)
+        off = ctypes.c_int32()
+        p = parameters.native()
+        st = self._lib.pairhmm_sw_align(self._ctx, r.ctypes.data, len(r), a.ctypes.data, len(a), ctypes.byref(p),
+                                        int(strategy), buf, cap, ctypes.byref(off))
+        _raise_for(st, "align")
+        return buf.value.decode("ascii"), off.value
+
+    def align_batch(self, refs: Sequence[bytes], alts: Sequence[bytes], parameters: SWParameters,
+                    strategy: SWOverhangStrategy) -> List[Tuple[str, int]]:
+        n = len(refs)
+        if n != len(alts):
+            raise ValueError("refs and alts differ in length")
+        if n == 0:
+            return []
+        rl = np.array([len(x) for x in refs], np.int32)
+        al = np.array([len(x) for x in alts], np.int32)
+        ro = np.concatenate([[0], np.cumsum(rl, dtype=np.int64)[:-1]]).astype(np.int64)
+        ao = np.concatenate([[0], np.cumsum(al, dtype=np.int64)[:-1]]).astype(np.int64)
+        rb = np.frombuffer(b"".join(bytes(x) for x in refs), np.uint8).copy()
+        ab = np.frombuffer(b"".join(bytes(x) for x in alts), np.uint8).copy()
+        caps = np.array([_cigar_cap(int(x), int(y)) for x, y in zip(rl, al)], np.int32)
+        coff = np.concatenate([[0], np.cumsum(caps, dtype=np.int64)[:-1]]).astype(np.int64)
+        cig = np.zeros(int(caps.sum()), np.uint8)
+        offs = np.zeros(n, np.int32)
+        p = parameters.native()
+        st = self._lib.pairhmm_sw_align_batch(self._ctx, n, rb.ctypes.data, ro.ctypes.data, rl.ctypes.data,
+                                              ab.ctypes.data, ao.ctypes.data, al.ctypes.data, ctypes.byref(p),
+                                              int(strategy), cig.ctypes.data, coff.ctypes.data, caps.ctypes.data,
+                                              offs.ctypes.data)
+        _raise_for(st, "align_batch")
+        out = []
+        for k in range(n):
+            s = cig[coff[k]:coff[k] + caps[k]].tobytes()
+            out.append((s[:s.index(0)].decode("ascii"), int(offs[k])))
+        return out
+
+    def close(self) -> None:
+        if self._lib is not None and self._ctx:
+            self._lib.pairhmm_done(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+
+class SWNativeAlignerWrapper:
+    """SWNativeAlignerWrapper (SWNativeAlignerWrapper.java:19-60) around the gfx950 binding."""
+
+    def __init__(self, binding: Optional[HipSmithWatermanBinding] = None) -> None:
+        self.aligner = binding or HipSmithWatermanBinding()
+        if binding is None and not self.aligner.load():
+            from .pairhmm import HardwareFeatureException
+            raise HardwareFeatureException("Machine does not support the gfx950 SmithWaterman.")
+
+    def align(self, reference, alternate, parameters: SWParameters,
+              overhang_strategy: SWOverhangStrategy) -> SmithWatermanAlignment:
+        ref, alt = bytes(_as_u8(reference)), bytes(_as_u8(alternate))
+        if overhang_strategy in (SWOverhangStrategy.SOFTCLIP, SWOverhangStrategy.IGNORE):
+            k = last_index_of(ref, alt)
+            if k != -1:
+                return SmithWatermanAlignment(f"{len(alt)}M", k)
+        cigar, off = self.aligner.align(ref, alt, parameters, overhang_strategy)
+        return SmithWatermanAlignment(cigar, off)
+
+    def close(self) -> None:
+        self.aligner.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

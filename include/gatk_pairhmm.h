@@ -310,6 +310,43 @@ int32_t pairhmm_normalize_and_filter_likelihoods(pairhmm_context *ctx, double *l
                                                  int32_t dynamic_disqualification, double disqualification_scale,
                                                  uint8_t *keep, double *threshold);
 
+/* ---- Smith-Waterman (drop-in for GKL's SWAlignerNativeBinding) ----------------------------- */
+
+/* SWParameters (gatk-native-bindings; SmithWatermanAlignmentConstants.java:30-68 holds the presets). */
+typedef struct pairhmm_sw_parameters {
+    int32_t match_value;
+    int32_t mismatch_penalty;
+    int32_t gap_open_penalty;
+    int32_t gap_extend_penalty;
+} pairhmm_sw_parameters;
+
+/* SWOverhangStrategy, in the binding's declaration order. */
+typedef enum pairhmm_sw_overhang_strategy {
+    PAIRHMM_SW_SOFTCLIP = 0,
+    PAIRHMM_SW_INDEL = 1,
+    PAIRHMM_SW_LEADING_INDEL = 2,
+    PAIRHMM_SW_IGNORE = 3
+} pairhmm_sw_overhang_strategy;
+
+/* SWAlignerNativeBinding.align(ref, alt, parameters, overhangStrategy) -> SWNativeAlignerResult
+ * (SWNativeAlignerWrapper.java:46; SmithWatermanIntelAligner.java:35): the CIGAR as text into cigar
+ * (cigar_cap bytes including the terminating 0) and the alignment offset.  The result is that of
+ * SmithWatermanJavaAligner's full alignment (SmithWatermanJavaAligner.java:104-377); the substring
+ * shortcut of the callers (SWNativeAlignerWrapper.java:35-43) is theirs to take, as with GKL.  Empty
+ * sequences are PAIRHMM_ERR_BAD_ARG (:55-57); sequences up to 32767 bases.  Synchronous. */
+int32_t pairhmm_sw_align(pairhmm_context *ctx, const uint8_t *ref, int32_t ref_len, const uint8_t *alt,
+                         int32_t alt_len, const pairhmm_sw_parameters *parameters, int32_t overhang_strategy,
+                         char *cigar, int32_t cigar_cap, int32_t *alignment_offset);
+
+/* Batch form (new; no counterpart in the reference): pair p aligns alt_bases[alt_off[p] ..] to
+ * ref_bases[ref_off[p] ..] with one set of parameters; its CIGAR text goes to cigars + cigar_off[p]
+ * (cigar_cap[p] bytes) and its offset to alignment_offset[p].  Host memory; one launch. */
+int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases,
+                               const int64_t *ref_off, const int32_t *ref_len, const uint8_t *alt_bases,
+                               const int64_t *alt_off, const int32_t *alt_len,
+                               const pairhmm_sw_parameters *parameters, int32_t overhang_strategy, char *cigars,
+                               const int64_t *cigar_off, const int32_t *cigar_cap, int32_t *alignment_offset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,8 @@ product path never does.
   (Log10PairHMM.java:58-183), the ``PairHMMModel`` / ``QualityUtils`` / ``JacobianLogTable``
   tables (PairHMMModel.java:71-117, 373-418; QualityUtils.java:43-57; MathUtils.java:406-480),
   plus an FP64 adjoint of the recurrence for continuous Phred parameters, and the partially
-  determined ``LoglessPDPairHMM`` (LoglessPDPairHMM.java:34-194).
+  determined ``LoglessPDPairHMM`` (LoglessPDPairHMM.java:34-194); ``sw_oracle.c`` restates
+  ``SmithWatermanJavaAligner`` (SmithWatermanJavaAligner.java:104-377).
 * ``torch_log10_likelihood`` is an independent anti-diagonal restatement of the same continuous
   model in torch FP64, used with ``torch.autograd.gradcheck`` to pin the C adjoint.
 """
@@ -70,6 +71,9 @@ def lib() -> ctypes.CDLL:
             L.oracle_pd_logless_batch.argtypes = [ctypes.c_int32, ctypes.c_int32] + [_u8p] * 5 + [
                 _i64p, _i32p, _u8p, _u8p, _i64p, _i32p, ctypes.c_int, ctypes.c_int, _dp]
             L.oracle_pd_logless_batch.restype = ctypes.c_int
+            L.oracle_sw_align.argtypes = [_u8p, ctypes.c_int, _u8p, ctypes.c_int] + [ctypes.c_int] * 5 + [
+                ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _i32p, _i32p]
+            L.oracle_sw_align.restype = ctypes.c_int
             L.oracle_log10.argtypes = pair + [ctypes.c_int, _dp]
             L.oracle_log10.restype = ctypes.c_int
             L.oracle_logless_batch.argtypes = [ctypes.c_int64, _i32p, _i32p, _u8p, _u8p, _u8p, _u8p, _u8p,
@@ -206,6 +210,30 @@ def pd_logless_batch(reads, haps, n_threads: int = 1, tristate: bool = True) -> 
     if st != 0:
         raise OracleError(f"oracle_pd_logless_batch status {st}")
     return out.reshape(len(rb), len(hb))
+
+
+SW_STRATEGIES = {"SOFTCLIP": 0, "INDEL": 1, "LEADING_INDEL": 2, "IGNORE": 3}
+
+
+def sw_align(ref, alt, params, strategy: str, matrices: bool = False):
+    """SmithWatermanJavaAligner's full alignment (no substring shortcut): (cigar text, offset), plus the
+    score and backtrack matrices ([R+1, A+1] int32) when ``matrices``.  ``params`` is SWParameters'
+    (match, mismatch, open, extend)."""
+    r, a = _bytes(ref), _bytes(alt)
+    cap = 16 * (len(r) + len(a) + 8)
+    buf = ctypes.create_string_buffer(cap)
+    off = ctypes.c_int()
+    sc = bt = None
+    if matrices:
+        sc = np.zeros((len(r) + 1, len(a) + 1), np.int32)
+        bt = np.zeros_like(sc)
+    st = lib().oracle_sw_align(_p(r), len(r), _p(a), len(a), *[int(x) for x in params], SW_STRATEGIES[strategy], buf,
+                               cap, ctypes.byref(off), _p(sc, _i32p) if matrices else None,
+                               _p(bt, _i32p) if matrices else None)
+    if st != 0:
+        raise OracleError(f"oracle_sw_align status {st}")
+    res = (buf.value.decode("ascii"), off.value)
+    return res + (sc, bt) if matrices else res
 
 
 def log10_pairhmm(hap, read, quals, ins, dels, gcp, exact: bool, tristate: bool = True) -> float:

@@ -1,0 +1,94 @@
+"""GPU parity of the Smith-Waterman kernel (pairhmm_sw_align / pairhmm_sw_align_batch, sw.hip)
+against the oracle: CIGAR text and alignment offset identical (integer arithmetic)."""
+import numpy as np
+import pytest
+
+from gatk_amd.smithwaterman import (ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS, NEW_SW_PARAMETERS, ORIGINAL_DEFAULT,
+                                    STANDARD_NGS, HipSmithWatermanBinding, SWNativeAlignerWrapper,
+                                    SWOverhangStrategy)
+from oracle import oracle
+from test_sw_oracle import CASES
+
+pytestmark = pytest.mark.gpu
+PRESETS = [ORIGINAL_DEFAULT, STANDARD_NGS, NEW_SW_PARAMETERS, ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS]
+
+
+@pytest.fixture(scope="module")
+def binding():
+    b = HipSmithWatermanBinding()
+    assert b.load()
+    yield b
+    b.close()
+
+
+def _p(p):
+    return (p.match_value, p.mismatch_penalty, p.gap_open_penalty, p.gap_extend_penalty)
+
+
+@pytest.mark.parametrize("ref,read,params,strategy,offset,cigar", CASES)
+def test_reference_cases(binding, ref, read, params, strategy, offset, cigar):
+    print(f"[sw] case {strategy} {len(ref)}x{len(read)}", flush=True)
+    w = SWNativeAlignerWrapper(binding)
+    a = w.align(ref.encode(), read.encode(), params, SWOverhangStrategy[strategy])
+    assert (a.cigar, a.alignment_offset) == (cigar, offset)
+    # The full alignment too (no substring shortcut), against the oracle.
+    assert binding.align(ref.encode(), read.encode(), params, SWOverhangStrategy[strategy]) == \
+        oracle.sw_align(ref.encode(), read.encode(), _p(params), strategy)
+
+
+def _read_from(rng, hap, n):
+    s = int(rng.integers(0, max(1, len(hap) - n)))
+    read = bytearray(hap[s:s + n])
+    for _ in range(int(rng.integers(0, 6))):
+        k = int(rng.integers(0, max(1, len(read))))
+        r = rng.random()
+        if r < 0.5:
+            if read:
+                read[k % len(read)] = b"ACGT"[int(rng.integers(0, 4))]
+        elif r < 0.75:
+            del read[k:k + int(rng.integers(1, 6))]
+        else:
+            read[k:k] = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, int(rng.integers(1, 6)))])
+    return bytes(read) or b"C"
+
+
+@pytest.mark.parametrize("strategy", list(SWOverhangStrategy))
+@pytest.mark.parametrize("preset", range(4))
+def test_batch_matches_oracle(binding, strategy, preset):
+    """Reads of 1-300 bases (stripe boundaries 63/64/65/128/129 included) against haplotypes of
+    1-700 bases, with substitutions and indels, plus unrelated pairs."""
+    rng = np.random.default_rng(100 * preset + int(strategy))
+    params = PRESETS[preset]
+    refs, alts = [], []
+    lens = [1, 2, 63, 64, 65, 127, 128, 129, 150, 151, 250, 300]
+    for k in range(160):
+        H = int(rng.integers(1, 700))
+        hap = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, H)])
+        n = lens[k % len(lens)] if k < 48 else int(rng.integers(1, 300))
+        alt = _read_from(rng, hap, n) if k % 5 else bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, n)])
+        refs.append(hap)
+        alts.append(alt)
+    print(f"[sw] batch strategy={strategy.name} preset={preset} pairs={len(refs)}", flush=True)
+    got = binding.align_batch(refs, alts, params, strategy)
+    for r, a, g in zip(refs, alts, got):
+        assert g == oracle.sw_align(r, a, _p(params), strategy.name), (len(r), len(a))
+
+
+def test_long_sequences(binding):
+    """Haplotype-to-reference shapes: 1-3 kb against each other (several 64-column stripes)."""
+    rng = np.random.default_rng(7)
+    refs, alts = [], []
+    for H in (1000, 2000, 3000):
+        ref = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, H)])
+        refs.append(ref)
+        alts.append(_read_from(rng, ref, H - 50))
+    got = binding.align_batch(refs, alts, NEW_SW_PARAMETERS, SWOverhangStrategy.SOFTCLIP)
+    for r, a, g in zip(refs, alts, got):
+        assert g == oracle.sw_align(r, a, _p(NEW_SW_PARAMETERS), "SOFTCLIP")
+
+
+def test_empty_sequence_is_an_error(binding):
+    with pytest.raises(ValueError):
+        binding.align(b"", b"ACGT", ORIGINAL_DEFAULT, SWOverhangStrategy.SOFTCLIP)
+    with pytest.raises(ValueError):
+        binding.align(b"ACGT", b"", ORIGINAL_DEFAULT, SWOverhangStrategy.INDEL)
