@@ -344,6 +344,52 @@ def postprocess_leg(args):
     return d
 
 
+def sw_leg(args):
+    """Read -> haplotype Smith-Waterman realignment (HaplotypeCaller's step after the PairHMM) through
+    the batch API: every read of batched_uneven regions against a haplotype of its region, with
+    ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS and SOFTCLIP, one call (host in / host out); the oracle
+    (one thread, C) on a sample as the CPU baseline, CIGARs compared on that sample."""
+    from gatk_amd import synth
+    from gatk_amd.smithwaterman import (ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS as P, HipSmithWatermanBinding,
+                                        SWOverhangStrategy)
+    from oracle import oracle
+
+    b = synth.workload("batched_uneven", n_regions=args.sw_regions, seed=31)
+    rng = np.random.default_rng(31)
+    refs, alts = [], []
+    for k in range(len(b.region_read_off) - 1):
+        h0, h1 = int(b.region_hap_off[k]), int(b.region_hap_off[k + 1])
+        for r in range(int(b.region_read_off[k]), int(b.region_read_off[k + 1])):
+            h = int(rng.integers(h0, h1))
+            refs.append(b.hap_bases[int(b.hap_off[h]):int(b.hap_off[h]) + int(b.hap_len[h])].tobytes())
+            alts.append(b.read_bases[int(b.read_off[r]):int(b.read_off[r]) + int(b.read_len[r])].tobytes())
+    cells = sum(len(r) * len(a) for r, a in zip(refs, alts))
+    sw = HipSmithWatermanBinding()
+    sw.load()
+    sw.align_batch(refs[:256], alts[:256], P, SWOverhangStrategy.SOFTCLIP)  # warm-up
+    t0 = time.perf_counter()
+    got = sw.align_batch(refs, alts, P, SWOverhangStrategy.SOFTCLIP)
+    wall = time.perf_counter() - t0
+    sw.close()
+    sample = rng.choice(len(refs), size=min(args.sw_sample, len(refs)), replace=False)
+    pp = (P.match_value, P.mismatch_penalty, P.gap_open_penalty, P.gap_extend_penalty)
+    s_cells, same = 0, True
+    c0 = time.perf_counter()
+    for k in sample:
+        same &= oracle.sw_align(refs[k], alts[k], pp, "SOFTCLIP") == got[k]
+        s_cells += len(refs[k]) * len(alts[k])
+    c1 = time.perf_counter()
+    d = {"alignments": len(refs), "cells": cells, "wall_s": round(wall, 3),
+         "alignments_per_s": round(len(refs) / wall, 1), "wall_gcups": round(cells / wall / 1e9, 2),
+         "matches_oracle_sample": bool(same),
+         "cpu_oracle_gcups": round(s_cells / (c1 - c0) / 1e9, 4),
+         "cpu_sample": f"SW oracle (C, 1 thread) on {len(sample)} alignments ({c1 - c0:.2f} s)",
+         "workload": f"every read of batched_uneven {args.sw_regions} regions (seed 31) against a random haplotype of "
+                     f"its region, parameters (10, -15, -30, -5), SOFTCLIP; one batch call, host in / host out"}
+    log(f"smith-waterman: {d}")
+    return d
+
+
 def chunked_leg(args, chunk_regions=500):
     """The streaming workload's region mix (90% batched_uneven-shaped, 10% STR-rich precision_stress
     regions), generated chunk by chunk in this process and run through the batch API (upload +
@@ -490,6 +536,9 @@ def main() -> int:
     ap.add_argument("--post-reads", type=int, default=1_000_000,
                     help="reads of the likelihood post-processing leg (0 skips it)")
     ap.add_argument("--post-sample", type=int, default=20_000)
+    ap.add_argument("--sw-regions", type=int, default=1_000,
+                    help="regions of the Smith-Waterman realignment leg (0 skips it)")
+    ap.add_argument("--sw-sample", type=int, default=2_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB read + 1 GiB write elementwise kernel (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -604,6 +653,9 @@ def main() -> int:
     post = None
     if args.post_reads > 0:
         post = postprocess_leg(args)
+    swr = None
+    if args.sw_regions > 0:
+        swr = sw_leg(args)
     precision = None
     if args.precision_regions > 0:
         precision = precision_leg(args, min(16, os.cpu_count() or 1))
@@ -641,6 +693,7 @@ def main() -> int:
         "read_prep": readprep,
         "pd_hmm": pdhmm,
         "likelihood_postprocessing": post,
+        "smith_waterman": swr,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
