@@ -12,7 +12,7 @@
 //      DRAGEN dynamic threshold is on, which takes the lower of that and
 //      -0.1 (sum mean_q + K sqrt(sum var_q)) over the read's HMM base qualities).
 // One thread per read over its row of the read-major matrix; FP64 with contraction off, so every
-// sum and product rounds where Java's do.
+// sum and product rounds where Java's do (the quality sums run in read order).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -39,6 +39,11 @@ __constant__ double kDynVar[40] = {
 
 __global__ void postprocess_kernel(PostprocessArgs p) {
 #pragma clang fp contract(off)
+    // The table in LDS: per-base lookups differ lane to lane, which LDS serves far better than
+    // gathers from global memory.
+    __shared__ double2 tab[40];
+    if (threadIdx.x < 40) tab[threadIdx.x] = make_double2(kDynMean[threadIdx.x], kDynVar[threadIdx.x]);
+    __syncthreads();
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= p.n_reads) return;
     const int A = p.n_alleles;
@@ -60,14 +65,26 @@ __global__ void postprocess_kernel(PostprocessArgs p) {
     const double errors = ceil(static_cast<double>(L) * p.max_error_per_base);
     double t = (p.dynamic ? errors : fmin(2.0, errors)) * -4.0;
     if (p.dynamic) {
+        // Sums in read order, as the reference adds them; the qualities are fetched 8 at a time
+        // from aligned words wholly inside the read, byte by byte at the two ends.
         const uint8_t *q = p.base_quals + p.read_off[r];
         double s_mean = 0.0, s_var = 0.0;
-        for (int i = 0; i < L; ++i) {
-            const int bq = q[i];
+        auto add = [&](int bq) {
             const int e = bq <= 1 ? 0 : (bq < 40 ? bq : 40) - 1;
-            s_mean += kDynMean[e];
-            s_var += kDynVar[e];
+            const double2 mv = tab[e];
+            s_mean += mv.x;
+            s_var += mv.y;
+        };
+        const int head = static_cast<int>((8u - (reinterpret_cast<uintptr_t>(q) & 7u)) & 7u);
+        const int h = head < L ? head : L;
+        int i = 0;
+        for (; i < h; ++i) add(q[i]);
+        for (; i + 8 <= L; i += 8) {
+            const uint64_t w = *reinterpret_cast<const uint64_t *>(q + i);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) add(static_cast<int>((w >> (8 * k)) & 0xFFu));
         }
+        for (; i < L; ++i) add(q[i]);
         const double d = (s_mean + p.scale * sqrt(s_var)) * -0.1;
         if (d < t) t = d;
     }
