@@ -16,9 +16,9 @@
 // wave_shr:1, and the cell to the left one step earlier is the diagonal.  Lane 0 takes them from an
 // LDS record per row that lane 63 of the previous stripe wrote (stripe 0: column 0).  Backtrack codes
 // (int16) go to the wave's global scratch in step-major order, 128 coalesced bytes per step; the
-// last column and the bottom row of scores go there too.  Lane 0 then picks the end point and walks
-// the backtrack exactly as calculateCigar does, writing the CIGAR elements in the reference's list
-// order (the host reverses them into text).
+// last column and the bottom row of scores go there too.  The wave then picks the end point and walks
+// the backtrack as calculateCigar does, 64 diagonal cells per fetch, writing the CIGAR elements in
+// the reference's list order (the host reverses them into text).
 #include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
 
@@ -45,10 +45,6 @@ __device__ __forceinline__ int32_t shr1_keep_i(int32_t old, int32_t src) {
 // :133-149), 0 otherwise.
 __device__ __forceinline__ int32_t edge_score(int p, bool gap_edges, int32_t open, int32_t extend) {
     return (gap_edges && p >= 1) ? open + (p - 1) * extend : 0;
-}
-
-__device__ __forceinline__ void emit(uint32_t *elems, int &n, uint32_t op, int32_t len) {
-    elems[n++] = (static_cast<uint32_t>(len) << 2) | op;
 }
 
 __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) {
@@ -163,25 +159,34 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (lane == 0) {
-            // calculateCigar (:256-377)
-            int p1 = 0, p2 = 0, segment = 0;
-            if (b.strategy == 1) {
-                p1 = R;
-                p2 = A;
-            } else {
-                int32_t best = INT32_MIN;
-                p2 = A;
-                for (int i = 1; i <= R; ++i) {
-                    const int32_t sc = lastcol[i];
+        // calculateCigar (:256-377), run by the whole wave on wave-uniform state: lanes fetch 64 end
+        // scores or 64 backtrack codes at a time, and every lane replays the reference's sequential
+        // decisions on them; lane 0 stores.
+        int p1 = 0, p2 = 0, segment = 0;
+        if (b.strategy == 1) {
+            p1 = R;
+            p2 = A;
+        } else {
+            int32_t best = INT32_MIN;
+            p2 = A;
+            for (int i0 = 1; i0 <= R; i0 += kWave) {
+                const int32_t mine = i0 + lane <= R ? lastcol[i0 + lane] : INT32_MIN;
+                const int cnt = R - i0 + 1 < kWave ? R - i0 + 1 : kWave;
+                for (int k = 0; k < cnt; ++k) {
+                    const int32_t sc = __shfl(mine, k, kWave);
                     if (sc >= best) {
-                        p1 = i;
+                        p1 = i0 + k;
                         best = sc;
                     }
                 }
-                if (b.strategy != 2) {
-                    for (int jj = 1; jj <= A; ++jj) {
-                        const int32_t sc = bottom[jj];
+            }
+            if (b.strategy != 2) {
+                for (int j0 = 1; j0 <= A; j0 += kWave) {
+                    const int32_t mine = j0 + lane <= A ? bottom[j0 + lane] : INT32_MIN;
+                    const int cnt = A - j0 + 1 < kWave ? A - j0 + 1 : kWave;
+                    for (int k = 0; k < cnt; ++k) {
+                        const int32_t sc = __shfl(mine, k, kWave);
+                        const int jj = j0 + k;
                         const int d_new = R - jj < 0 ? jj - R : R - jj;
                         const int d_old = p1 - p2 < 0 ? p2 - p1 : p1 - p2;
                         if (sc > best || (sc == best && d_new < d_old)) {
@@ -193,60 +198,74 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
                     }
                 }
             }
-            uint32_t *elems = b.elems + b.elem_off[p];
-            int n = 0;
-            if (segment > 0 && b.strategy == 0) {
-                emit(elems, n, kOpS, segment);
-                segment = 0;
+        }
+        uint32_t *elems = b.elems + b.elem_off[p];
+        int n = 0;
+        auto put = [&](uint32_t op, int32_t len) {
+            if (lane == 0) elems[n] = (static_cast<uint32_t>(len) << 2) | op;
+            ++n;
+        };
+        if (segment > 0 && b.strategy == 0) {
+            put(kOpS, segment);
+            segment = 0;
+        }
+        uint32_t state = kOpM;
+        do {
+            // The next 64 cells down the diagonal from (p1, p2): a run of diagonal codes is a run of
+            // single MATCH steps, taken at once; the first other code is one gap step.
+            const int ci = p1 - lane, cj = p2 - lane;
+            int code = 1;  // outside the matrix: stop
+            if (ci >= 1 && cj >= 1) {
+                const int ss = (cj - 1) / kWave, ll = (cj - 1) % kWave;
+                code = bt[(static_cast<size_t>(ss) * steps + (ci - 1 + ll)) * kWave + ll];
             }
-            uint32_t state = kOpM;
-            do {
-                const int ss = (p2 - 1) / kWave, ll = (p2 - 1) % kWave;
-                const int tt = p1 - 1 + ll;
-                const int btr = bt[(static_cast<size_t>(ss) * steps + tt) * kWave + ll];
-                uint32_t ns;
-                int step = 1;
-                if (btr > 0) {
-                    ns = kOpD;
-                    step = btr;
-                } else if (btr < 0) {
-                    ns = kOpI;
-                    step = -btr;
+            const uint64_t stop = __ballot(code != 0);
+            const int f = stop ? __builtin_ctzll(stop) : kWave;
+            if (f > 0) {
+                if (state == kOpM) {
+                    segment += f;
                 } else {
-                    ns = kOpM;
+                    if (segment > 0) put(state, segment);
+                    segment = f;
+                    state = kOpM;
                 }
-                if (ns == kOpM) {
-                    --p1;
-                    --p2;
-                } else if (ns == kOpI) {
+                p1 -= f;
+                p2 -= f;
+            }
+            if (f < kWave && p1 > 0 && p2 > 0) {
+                const int btr = __shfl(code, f, kWave);
+                const uint32_t ns = btr > 0 ? kOpD : kOpI;
+                const int step = btr > 0 ? btr : -btr;
+                if (ns == kOpI)
                     p2 -= step;
-                } else {
+                else
                     p1 -= step;
-                }
                 if (ns == state) {
                     segment += step;
                 } else {
-                    if (segment > 0) emit(elems, n, state, segment);
+                    if (segment > 0) put(state, segment);
                     segment = step;
                     state = ns;
                 }
-            } while (p1 > 0 && p2 > 0);
-            int offset;
-            if (b.strategy == 0) {
-                emit(elems, n, state, segment);
-                if (p2 > 0) emit(elems, n, kOpS, p2);
-                offset = p1;
-            } else if (b.strategy == 3) {
-                emit(elems, n, state, segment + p2);
-                offset = p1 - p2;
-            } else {
-                emit(elems, n, state, segment);
-                if (p1 > 0)
-                    emit(elems, n, kOpD, p1);
-                else if (p2 > 0)
-                    emit(elems, n, kOpI, p2);
-                offset = 0;
             }
+        } while (p1 > 0 && p2 > 0);
+        int offset;
+        if (b.strategy == 0) {
+            put(state, segment);
+            if (p2 > 0) put(kOpS, p2);
+            offset = p1;
+        } else if (b.strategy == 3) {
+            put(state, segment + p2);
+            offset = p1 - p2;
+        } else {
+            put(state, segment);
+            if (p1 > 0)
+                put(kOpD, p1);
+            else if (p2 > 0)
+                put(kOpI, p2);
+            offset = 0;
+        }
+        if (lane == 0) {
             b.n_elems[p] = n;
             b.offset[p] = offset;
         }
