@@ -11,8 +11,9 @@
 //      dropped when its best likelihood is below ceil(L * rate) * -4 (at most 2 errors unless the
 //      DRAGEN dynamic threshold is on, which takes the lower of that and
 //      -0.1 (sum mean_q + K sqrt(sum var_q)) over the read's HMM base qualities).
-// One thread per read over its row of the read-major matrix; FP64 with contraction off, so every
-// sum and product rounds where Java's do (the quality sums run in read order).
+// One thread per read; a wave's 64 matrix rows and quality strings are staged through LDS with
+// coalesced loads.  FP64 with contraction off, so every sum and product rounds where Java's do (the
+// quality sums run in read order).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -37,18 +38,44 @@ __constant__ double kDynVar[40] = {
     2.791361596, 2.370765375, 2.008921719, 1.698687797, 1.433525748, 1.207526336, 1.015402928, 0.852465956,
     0.714585285, 0.598145851, 0.500000349, 0.41742159,  0.348056286, 0.289881373, 0.241163527, 0.200422214};
 
-__global__ void postprocess_kernel(PostprocessArgs p) {
+constexpr int kPpWaves = 2;             // waves per workgroup, each over 64 consecutive reads
+constexpr int kQualStage = 10240;       // bytes of HBM qualities a wave stages in LDS
+constexpr int kRowStage = 512;          // likelihoods a wave stages in LDS (64 reads x 8 haplotypes)
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kPpWaves * 64) void postprocess_kernel(PostprocessArgs p) {
 #pragma clang fp contract(off)
     // The table in LDS: per-base lookups differ lane to lane, which LDS serves far better than
     // gathers from global memory.
     __shared__ double2 tab[40];
+    // Each wave's reads are adjacent in the matrix and (usually) in the quality plane: both are
+    // brought into LDS with coalesced loads, worked on there, and the rows written back coalesced.
+    __shared__ uint32_t qstage[kPpWaves][kQualStage / 4];
+    __shared__ double rstage[kPpWaves][kRowStage];
     if (threadIdx.x < 40) tab[threadIdx.x] = make_double2(kDynMean[threadIdx.x], kDynVar[threadIdx.x]);
     __syncthreads();
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= p.n_reads) return;
+    const int w = static_cast<int>(threadIdx.x) / 64, lane = static_cast<int>(threadIdx.x) % 64;
+    const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * kPpWaves + w) * 64;
+    if (r0 >= p.n_reads) return;  // the whole wave
+    const int nr = p.n_reads - r0 < 64 ? static_cast<int>(p.n_reads - r0) : 64;
+    const bool has = lane < nr;
+    const int64_t r = r0 + lane;
     const int A = p.n_alleles;
-    double *v = p.likelihoods + r * A;
-    if (p.normalize && A >= 2) {
+
+    const bool stage_rows = static_cast<int64_t>(nr) * A <= kRowStage;
+    double *const grow = p.likelihoods + r0 * A;
+    if (stage_rows) {
+        for (int e = lane; e < nr * A; e += 64) rstage[w][e] = grow[e];
+        wave_lds_sync();
+    }
+    double *v = !has ? rstage[w] : stage_rows ? &rstage[w][lane * A] : p.likelihoods + r * A;
+
+    if (has && p.normalize && A >= 2) {
         const bool can_ref = p.symmetric != 0;
         const int b0 = (can_ref || p.ref_index != 0) ? 0 : 1;
         double best = v[b0];
@@ -61,47 +88,72 @@ __global__ void postprocess_kernel(PostprocessArgs p) {
         for (int a = 0; a < A; ++a)
             if (v[a] < worst) v[a] = worst;
     }
-    const int L = p.read_len[r];
+    const int L = has ? p.read_len[r] : 0;
     const double errors = ceil(static_cast<double>(L) * p.max_error_per_base);
     double t = (p.dynamic ? errors : fmin(2.0, errors)) * -4.0;
     if (p.dynamic) {
-        // Sums in read order, as the reference adds them; the qualities are fetched 8 at a time
-        // from aligned words wholly inside the read, byte by byte at the two ends.
-        const uint8_t *q = p.base_quals + p.read_off[r];
+        // The wave's quality span [lo, hi) in global addresses; staged when it fits, read in place
+        // otherwise.  Sums run in read order, as the reference adds them.
+        const uint8_t *q = p.base_quals + (has ? p.read_off[r] : 0);
+        uint64_t lo = has ? reinterpret_cast<uintptr_t>(q) : ~0ull, hi = has ? lo + L : 0ull;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t olo = __shfl_xor(static_cast<unsigned long long>(lo), off, 64);
+            const uint64_t ohi = __shfl_xor(static_cast<unsigned long long>(hi), off, 64);
+            lo = olo < lo ? olo : lo;
+            hi = ohi > hi ? ohi : hi;
+        }
+        const uint64_t blo = lo & ~3ull;
+        const bool stage_q = hi > lo && hi - blo <= static_cast<uint64_t>(kQualStage);
+        const uint8_t *src = q;
+        if (stage_q) {
+            uint8_t *qb = reinterpret_cast<uint8_t *>(qstage[w]);
+            const uint64_t wlo = (lo + 3) & ~3ull, whi = hi & ~3ull;
+            const uint8_t *gb = reinterpret_cast<const uint8_t *>(blo);
+            if (wlo < whi) {
+                const uint32_t *gw = reinterpret_cast<const uint32_t *>(wlo);
+                const int nw = static_cast<int>((whi - wlo) / 4), w0 = static_cast<int>((wlo - blo) / 4);
+                for (int k = lane; k < nw; k += 64) qstage[w][w0 + k] = gw[k];
+                const int head = static_cast<int>(wlo - lo), tail = static_cast<int>(hi - whi);
+                if (lane < head) qb[lo - blo + lane] = gb[lo - blo + lane];
+                if (lane < tail) qb[whi - blo + lane] = gb[whi - blo + lane];
+            } else {
+                for (int k = lane; k < static_cast<int>(hi - lo); k += 64) qb[lo - blo + k] = gb[lo - blo + k];
+            }
+            wave_lds_sync();
+            src = qb + (reinterpret_cast<uintptr_t>(q) - blo);
+        }
         double s_mean = 0.0, s_var = 0.0;
-        auto add = [&](int bq) {
+        for (int i = 0; i < L; ++i) {
+            const int bq = src[i];
             const int e = bq <= 1 ? 0 : (bq < 40 ? bq : 40) - 1;
             const double2 mv = tab[e];
             s_mean += mv.x;
             s_var += mv.y;
-        };
-        const int head = static_cast<int>((8u - (reinterpret_cast<uintptr_t>(q) & 7u)) & 7u);
-        const int h = head < L ? head : L;
-        int i = 0;
-        for (; i < h; ++i) add(q[i]);
-        for (; i + 8 <= L; i += 8) {
-            const uint64_t w = *reinterpret_cast<const uint64_t *>(q + i);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) add(static_cast<int>((w >> (8 * k)) & 0xFFu));
         }
-        for (; i < L; ++i) add(q[i]);
         const double d = (s_mean + p.scale * sqrt(s_var)) * -0.1;
         if (d < t) t = d;
     }
-    double m = -__builtin_inf();
-    for (int a = 0; a < A; ++a)
-        if (v[a] > m) m = v[a];
-    p.keep[r] = (m < t) ? 0 : 1;
-    if (p.threshold) p.threshold[r] = t;
+    if (has) {
+        double m = -__builtin_inf();
+        for (int a = 0; a < A; ++a)
+            if (v[a] > m) m = v[a];
+        p.keep[r] = (m < t) ? 0 : 1;
+        if (p.threshold) p.threshold[r] = t;
+    }
+    if (stage_rows && p.normalize && A >= 2) {
+        wave_lds_sync();
+        for (int e = lane; e < nr * A; e += 64) grow[e] = rstage[w][e];
+    }
 }
 
 }  // namespace
 
 hipError_t launch_postprocess(const PostprocessArgs &args, hipStream_t stream) {
     if (args.n_reads <= 0) return hipSuccess;
-    const int block = 256;
-    const int64_t grid = (args.n_reads + block - 1) / block;
-    hipLaunchKernelGGL(postprocess_kernel, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, args);
+    const int64_t per_block = 64 * kPpWaves;
+    const int64_t grid = (args.n_reads + per_block - 1) / per_block;
+    hipLaunchKernelGGL(postprocess_kernel, dim3(static_cast<unsigned>(grid)), dim3(per_block), 0, stream, args);
     return hipGetLastError();
 }
 
