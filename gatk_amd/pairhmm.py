@@ -86,6 +86,45 @@ class HaplotypeDataHolder:
     haplotype_bases: np.ndarray
 
 
+# numpy images of the C-ABI holder structs (5 pointers + length; 1 pointer + length), so a whole
+# call's holders are filled with a few vectorised assignments instead of one ctypes struct per read.
+READ_HOLDER_DTYPE = np.dtype([("read_bases", "<u8"), ("read_quals", "<u8"), ("insertion_gop", "<u8"),
+                              ("deletion_gop", "<u8"), ("overall_gcp", "<u8"), ("length", "<i4"), ("_pad", "<i4")])
+HAPLOTYPE_HOLDER_DTYPE = np.dtype([("haplotype_bases", "<u8"), ("length", "<i4"), ("_pad", "<i4")])
+
+
+def _u8_list(items) -> List[np.ndarray]:
+    return [x if isinstance(x, np.ndarray) and x.dtype == np.uint8 and x.ndim == 1 else _as_u8(x) for x in items]
+
+
+def marshal_planes(dtype: np.dtype, columns):
+    """Concatenates each column (a list of per-item byte arrays of equal lengths item by item) into
+    one contiguous plane and returns a holder array of ``dtype`` whose pointer fields (in field
+    order) address each item's bytes, plus the planes to keep alive for the call."""
+    cols = [_u8_list(c) for c in columns]
+    lens = np.fromiter((len(x) for x in cols[0]), dtype=np.int64, count=len(cols[0]))
+    for c in cols[1:]:
+        if not np.array_equal(np.fromiter((len(x) for x in c), dtype=np.int64, count=len(c)), lens):
+            raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
+    offs = np.zeros(len(lens), dtype=np.uint64)
+    if len(lens) > 1:
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    holders = np.zeros(len(lens), dtype=dtype)
+    planes = []
+    for name, c in zip(dtype.names, cols):
+        plane = np.concatenate(c) if len(c) and lens.sum() else np.zeros(1, np.uint8)
+        planes.append(plane)
+        holders[name] = np.uint64(plane.ctypes.data) + offs
+    holders["length"] = lens
+    return holders, planes
+
+
+def marshal_reads(reads):
+    return marshal_planes(READ_HOLDER_DTYPE, [[r.read_bases for r in reads], [r.read_quals for r in reads],
+                                              [r.insertion_gop for r in reads], [r.deletion_gop for r in reads],
+                                              [r.overall_gcp for r in reads]])
+
+
 class HipPairHmm:
     """The native binding: PairHMMNativeBinding's load / initialize / computeLikelihoods / done."""
 
@@ -107,26 +146,14 @@ class HipPairHmm:
         """computeLikelihoods(reads, haplotypes, out): out[r * n_haps + h], read-major."""
         if len(reads) == 0 or len(haplotypes) == 0:
             return
-        keep = []
-        rd = (_native.ReadDataHolder * len(reads))()
-        for k, r in enumerate(reads):
-            arrs = [np.ascontiguousarray(_as_u8(a)) for a in
-                    (r.read_bases, r.read_quals, r.insertion_gop, r.deletion_gop, r.overall_gcp)]
-            n = len(arrs[0])
-            if any(len(a) != n for a in arrs):
-                raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
-            keep.extend(arrs)
-            rd[k] = _native.ReadDataHolder(*(a.ctypes.data for a in arrs), n)
-        hd = (_native.HaplotypeDataHolder * len(haplotypes))()
-        for k, h in enumerate(haplotypes):
-            a = np.ascontiguousarray(_as_u8(h.haplotype_bases))
-            keep.append(a)
-            hd[k] = _native.HaplotypeDataHolder(a.ctypes.data, len(a))
         out = likelihoods
         if out.dtype != np.float64 or not out.flags.c_contiguous or out.size < len(reads) * len(haplotypes):
             raise ValueError("likelihoods must be a contiguous float64 array of n_reads * n_haplotypes")
-        st = self._lib.pairhmm_compute_likelihoods(self._ctx, rd, len(reads), hd, len(haplotypes),
-                                                   out.ctypes.data)
+        rd, keep = marshal_reads(reads)
+        hd, keep2 = marshal_planes(HAPLOTYPE_HOLDER_DTYPE, [[h.haplotype_bases for h in haplotypes]])
+        st = self._lib.pairhmm_compute_likelihoods(
+            self._ctx, rd.ctypes.data_as(ctypes.POINTER(_native.ReadDataHolder)), len(reads),
+            hd.ctypes.data_as(ctypes.POINTER(_native.HaplotypeDataHolder)), len(haplotypes), out.ctypes.data)
         _raise_for(st, "computeLikelihoods")
 
     def done(self) -> None:

@@ -16,7 +16,11 @@ from typing import Callable, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native
-from .pairhmm import (LikelihoodMatrix, ReadDataHolder, _as_u8, _raise_for, java_format_e, phred_to_fastq)
+from .pairhmm import (LikelihoodMatrix, ReadDataHolder, _as_u8, _raise_for, java_format_e, marshal_planes,
+                      marshal_reads, phred_to_fastq)
+
+PD_HAPLOTYPE_HOLDER_DTYPE = np.dtype([("haplotype_bases", "<u8"), ("haplotype_pd_bases", "<u8"), ("length", "<i4"),
+                                      ("_pad", "<i4")])
 
 # PartiallyDeterminedHaplotype byte layout (PartiallyDeterminedHaplotype.java:59-66).
 SNP, DEL_START, DEL_END = 1, 2, 4
@@ -80,26 +84,16 @@ class HipPDHmm:
         if (likelihoods.dtype != np.float64 or not likelihoods.flags.c_contiguous
                 or likelihoods.size < len(reads) * len(haplotypes)):
             raise ValueError("likelihoods must be a contiguous float64 array of n_reads * n_haplotypes")
-        keep = []
-        rd = (_native.ReadDataHolder * len(reads))()
-        for k, r in enumerate(reads):
-            arrs = [np.ascontiguousarray(_as_u8(a)) for a in
-                    (r.read_bases, r.read_quals, r.insertion_gop, r.deletion_gop, r.overall_gcp)]
-            n = len(arrs[0])
-            if any(len(a) != n for a in arrs):
-                raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
-            keep.extend(arrs)
-            rd[k] = _native.ReadDataHolder(*(a.ctypes.data for a in arrs), n)
-        hd = (_native.PDHaplotypeDataHolder * len(haplotypes))()
-        for k, h in enumerate(haplotypes):
-            b = np.ascontiguousarray(_as_u8(h.haplotype_bases))
-            p = np.ascontiguousarray(_as_u8(h.haplotype_pd_bases))
-            if len(b) != len(p):
-                raise ValueError("haplotype bases and PD bases aren't the same size")
-            keep.extend((b, p))
-            hd[k] = _native.PDHaplotypeDataHolder(b.ctypes.data, p.ctypes.data, len(b))
-        st = self._lib.pairhmm_pd_compute_likelihoods(self._ctx, rd, len(reads), hd, len(haplotypes),
-                                                      likelihoods.ctypes.data)
+        rd, keep = marshal_reads(reads)
+        try:
+            hd, keep2 = marshal_planes(PD_HAPLOTYPE_HOLDER_DTYPE, [[h.haplotype_bases for h in haplotypes],
+                                                                   [h.haplotype_pd_bases for h in haplotypes]])
+        except ValueError:
+            raise ValueError("haplotype bases and PD bases aren't the same size") from None
+        st = self._lib.pairhmm_pd_compute_likelihoods(
+            self._ctx, rd.ctypes.data_as(ctypes.POINTER(_native.ReadDataHolder)), len(reads),
+            hd.ctypes.data_as(ctypes.POINTER(_native.PDHaplotypeDataHolder)), len(haplotypes),
+            likelihoods.ctypes.data)
         _raise_for(st, "computeLikelihoods")
 
     def done(self) -> None:
