@@ -193,6 +193,7 @@ struct PdBatch {
     int32_t n_haps;
     int32_t tristate;
     double *out;
+    const uint32_t *order;     // launch order of the pairs (longest first), or null
 };
 
 // LDS of one wave (one workgroup) for haplotypes up to max_hap_len: one 32-byte record per column.

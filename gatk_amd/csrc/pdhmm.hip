@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kWave) void pd_forward_kernel(PdBatch b, DeviceTabl
     extern __shared__ PdRecord rec[];
     const int lane = static_cast<int>(threadIdx.x);
     const double log10_init = log10(ldexp(1.0, Scale<double>::kExp));
-    for (uint32_t pair = blockIdx.x; pair < n_pairs; pair += gridDim.x) {
+    for (uint32_t k = blockIdx.x; k < n_pairs; k += gridDim.x) {
+        const uint32_t pair = b.order ? b.order[k] : k;
         const int r = static_cast<int>(pair / static_cast<uint32_t>(b.n_haps));
         const int h = static_cast<int>(pair % static_cast<uint32_t>(b.n_haps));
         const int R = b.rd_len[r], H = b.hap_len[h];

@@ -118,7 +118,8 @@ extern "C" int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pa
     const size_t o_rb = take(read_bytes), o_rq = take(read_bytes), o_ri = take(read_bytes), o_rd = take(read_bytes),
                  o_rg = take(read_bytes), o_hc = take(sizeof(uint32_t) * hap_cols),
                  o_roff = take(sizeof(int64_t) * n_reads), o_rlen = take(sizeof(int32_t) * n_reads),
-                 o_hoff = take(sizeof(int64_t) * n_haps), o_hlen = take(sizeof(int32_t) * n_haps);
+                 o_hoff = take(sizeof(int64_t) * n_haps), o_hlen = take(sizeof(int32_t) * n_haps),
+                 o_order = take(sizeof(uint32_t) * n_pairs);
     const size_t in_bytes = off;
     const size_t o_out = take(sizeof(double) * n_pairs);
     const size_t total = off;
@@ -154,10 +155,22 @@ extern "C" int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pa
         pd_column_codes(haps[h].haplotype_bases, haps[h].haplotype_pd_bases, haps[h].length, hcode + pos);
         pos += haps[h].length;
     }
+    // Workgroups are dispatched in order: longest reads first, and each read's haplotypes longest
+    // first, keeps the last wave of a launch short (outputs stay read-major).
+    uint32_t *order = reinterpret_cast<uint32_t *>(hs + o_order);
+    std::vector<int32_t> rs(n_reads), hsrt(n_haps);
+    for (int32_t r = 0; r < n_reads; ++r) rs[r] = r;
+    for (int32_t h = 0; h < n_haps; ++h) hsrt[h] = h;
+    std::stable_sort(rs.begin(), rs.end(), [&](int32_t a, int32_t c) { return rlen[a] > rlen[c]; });
+    std::stable_sort(hsrt.begin(), hsrt.end(), [&](int32_t a, int32_t c) { return hlen[a] > hlen[c]; });
+    for (int64_t k = 0, i = 0; i < n_reads; ++i)
+        for (int32_t j = 0; j < n_haps; ++j, ++k)
+            order[k] = static_cast<uint32_t>(static_cast<int64_t>(rs[i]) * n_haps + hsrt[j]);
     uint8_t *ds = static_cast<uint8_t *>(ctx->stage_dev.ptr);
     HIP_TRY(hipMemcpyAsync(ds, hs, in_bytes, hipMemcpyHostToDevice, ctx->stream), PAIRHMM_ERR_KERNEL);
 
     PdBatch b{};
+    b.order = reinterpret_cast<const uint32_t *>(ds + o_order);
     b.hap_code = reinterpret_cast<const uint32_t *>(ds + o_hc);
     b.hap_off = reinterpret_cast<const int64_t *>(ds + o_hoff);
     b.hap_len = reinterpret_cast<const int32_t *>(ds + o_hlen);
