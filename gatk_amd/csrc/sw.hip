@@ -13,8 +13,8 @@
 // column 64 s + k + 1 and at step t computes row i = t - k + 1 (anti-diagonal sweep over the
 // reference).  A lane keeps its column's vertical gap state and its previous row; the row's
 // horizontal gap state, the cell to the left and the reference base move lane to lane by DPP
-// wave_shr:1, and the cell to the left one step earlier is the diagonal.  Lane 0 takes them from an
-// LDS record per row that lane 63 of the previous stripe wrote (stripe 0: column 0).  Backtrack codes
+// wave_shr:1, and the cell to the left one step earlier is the diagonal.  Lane 0 takes them from a
+// 12-byte LDS record per row that lane 63 of the previous stripe wrote (stripe 0: column 0).  Backtrack codes
 // (int16) go to the wave's global scratch in step-major order, 128 coalesced bytes per step; the
 // last column and the bottom row of scores go there too.  The wave then picks the end point and walks
 // the backtrack as calculateCigar does, 64 diagonal cells per fetch, writing the CIGAR elements in
@@ -27,9 +27,10 @@ namespace {
 
 using namespace dev;
 
-struct alignas(16) SwRecord {
-    int32_t sw, bgh, gsh;
-    uint32_t base;
+// A row's boundary record, 12 bytes: the score and horizontal gap of the stripe's last column, and
+// the gap length with the row's reference base (gsh | base << 16; gap lengths stay below 32768).
+struct alignas(8) SwScore {
+    int32_t sw, bgh;
 };
 
 constexpr int kSwPad = 66;
@@ -48,7 +49,8 @@ __device__ __forceinline__ int32_t edge_score(int p, bool gap_edges, int32_t ope
 }
 
 __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) {
-    extern __shared__ SwRecord recs[];
+    extern __shared__ SwScore recs[];
+    uint32_t *gbase = nullptr;  // set per pair: (R + kSwPad) words after the score records
     const int lane = static_cast<int>(threadIdx.x);
     const int32_t w_match = b.match, w_mismatch = b.mismatch, w_open = b.open, w_extend = b.extend;
     const bool gap_edges = b.strategy == 1 || b.strategy == 2;
@@ -63,13 +65,13 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
         const int steps = R + kWave - 1;
         const int n_stripes = (A + kWave - 1) / kWave;
         __builtin_amdgcn_wave_barrier();
+        gbase = reinterpret_cast<uint32_t *>(recs + (R + kSwPad));
         for (int i = lane; i < R + kSwPad; i += kWave) {
-            SwRecord v;
+            SwScore v;
             v.sw = i <= R ? edge_score(i, gap_edges, w_open, w_extend) : 0;
             v.bgh = kLowInit;
-            v.gsh = 0;
-            v.base = (i >= 1 && i <= R) ? ref[i - 1] : 0u;
             recs[i] = v;
+            gbase[i] = (i >= 1 && i <= R) ? static_cast<uint32_t>(ref[i - 1]) << 16 : 0u;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -85,17 +87,20 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
             uint32_t a = 0;
             // Diagonal: lane 0 starts from row 0 of the previous stripe's last column.
             int32_t diag = lane == 0 ? edge_score(s * kWave, gap_edges, w_open, w_extend) : 0;
-            SwRecord nxt = recs[1];
+            SwScore nxt = recs[1];
+            uint32_t nxt_g = gbase[1];
             int16_t *bts = bt + static_cast<size_t>(s) * steps * kWave;
             for (int t = 0; t < steps; ++t) {
-                const SwRecord rc = nxt;
+                const SwScore rc = nxt;
+                const uint32_t rg = nxt_g;
                 nxt = recs[t + 2];
+                nxt_g = gbase[t + 2];
                 const int i = t - lane + 1;
                 // Values of column j-1 at row i (computed one step earlier by the lane to the left).
                 const int32_t left = shr1_keep_i(rc.sw, cur);
                 const int32_t l_bgh = shr1_keep_i(rc.bgh, bgh);
-                const int32_t l_gsh = shr1_keep_i(rc.gsh, gsh);
-                a = static_cast<uint32_t>(shr1_keep_i(static_cast<int32_t>(rc.base), static_cast<int32_t>(a)));
+                const int32_t l_gsh = shr1_keep_i(static_cast<int32_t>(rg & 0xFFFFu), gsh);
+                a = static_cast<uint32_t>(shr1_keep_i(static_cast<int32_t>(rg >> 16), static_cast<int32_t>(a)));
                 const bool active = col_ok && i >= 1 && i <= R;
                 const int32_t step_diag = diag + (a == bj ? w_match : w_mismatch);
                 int32_t prev_gap = up + w_open;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
                 if (lane == kWave - 1 && col_ok && i >= 1 && i <= R) {
                     recs[i].sw = v;
                     recs[i].bgh = n_bgh;
-                    recs[i].gsh = n_gsh;
+                    gbase[i] = (gbase[i] & 0xFFFF0000u) | static_cast<uint32_t>(n_gsh);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -275,7 +280,9 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
 
 }  // namespace
 
-size_t sw_lds_bytes(int max_ref_len) { return static_cast<size_t>(max_ref_len + kSwPad) * sizeof(SwRecord); }
+size_t sw_lds_bytes(int max_ref_len) {
+    return static_cast<size_t>(max_ref_len + kSwPad) * (sizeof(SwScore) + sizeof(uint32_t));
+}
 
 size_t sw_bt_bytes_per_wave(int max_ref_len, int max_alt_len) {
     const size_t stripes = static_cast<size_t>((max_alt_len + kWave - 1) / kWave);

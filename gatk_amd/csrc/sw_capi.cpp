@@ -88,9 +88,9 @@ extern "C" int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs,
     const size_t out_begin = o_elems, out_bytes = off - o_elems;
     const size_t per_wave = sw_scratch_bytes_per_wave(max_r, max_a);
     const int cus = std::max(ctx->num_cus, 1);
-    // 16 waves per CU: the LDS records of 600-base references allow it, and the lane-0 traceback
-    // is latency-bound, so more resident waves hide more of it.
-    int64_t waves = std::min<int64_t>(n_pairs, static_cast<int64_t>(cus) * 16);
+    // 20 waves per CU: the 12-byte LDS row records of 600-base references allow it, and the sweep's
+    // lane-to-lane dependency chain needs many resident waves to hide its latency.
+    int64_t waves = std::min<int64_t>(n_pairs, static_cast<int64_t>(cus) * 20);
     waves = std::max<int64_t>(1, std::min<int64_t>(waves, static_cast<int64_t>(kSwScratchBudget / per_wave)));
     const size_t o_scratch = take(per_wave * static_cast<size_t>(waves));
 
