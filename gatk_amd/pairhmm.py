@@ -93,26 +93,34 @@ READ_HOLDER_DTYPE = np.dtype([("read_bases", "<u8"), ("read_quals", "<u8"), ("in
 HAPLOTYPE_HOLDER_DTYPE = np.dtype([("haplotype_bases", "<u8"), ("length", "<i4"), ("_pad", "<i4")])
 
 
-def _u8_list(items) -> List[np.ndarray]:
-    return [x if isinstance(x, np.ndarray) and x.dtype == np.uint8 and x.ndim == 1 else _as_u8(x) for x in items]
+def _concat_u8(items) -> np.ndarray:
+    """The items' bytes as one contiguous uint8 plane (uint8 ndarrays are joined directly)."""
+    try:
+        plane = np.concatenate(items)
+        if plane.dtype == np.uint8 and plane.ndim == 1:
+            return plane
+    except (ValueError, TypeError):
+        pass
+    return np.concatenate([_as_u8(x) for x in items])
 
 
 def marshal_planes(dtype: np.dtype, columns):
     """Concatenates each column (a list of per-item byte arrays of equal lengths item by item) into
     one contiguous plane and returns a holder array of ``dtype`` whose pointer fields (in field
     order) address each item's bytes, plus the planes to keep alive for the call."""
-    cols = [_u8_list(c) for c in columns]
-    lens = np.fromiter((len(x) for x in cols[0]), dtype=np.int64, count=len(cols[0]))
-    for c in cols[1:]:
-        if not np.array_equal(np.fromiter((len(x) for x in c), dtype=np.int64, count=len(c)), lens):
+    n = len(columns[0])
+    lens = np.fromiter(map(len, columns[0]), dtype=np.int64, count=n)
+    for c in columns[1:]:
+        if not np.array_equal(np.fromiter(map(len, c), dtype=np.int64, count=n), lens):
             raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
-    offs = np.zeros(len(lens), dtype=np.uint64)
-    if len(lens) > 1:
+    offs = np.zeros(n, dtype=np.uint64)
+    if n > 1:
         offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
-    holders = np.zeros(len(lens), dtype=dtype)
+    holders = np.zeros(n, dtype=dtype)
     planes = []
-    for name, c in zip(dtype.names, cols):
-        plane = np.concatenate(c) if len(c) and lens.sum() else np.zeros(1, np.uint8)
+    empty = n == 0 or int(lens.sum()) == 0
+    for name, c in zip(dtype.names, columns):
+        plane = np.zeros(1, np.uint8) if empty else _concat_u8(c)
         planes.append(plane)
         holders[name] = np.uint64(plane.ctypes.data) + offs
     holders["length"] = lens
