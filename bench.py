@@ -59,6 +59,35 @@ def cpu_baseline(batch, out, seconds: float, threads: int):
                        f"(oracle/pairhmm_oracle.c), OpenMP over pairs, {t_used:.1f} s"), worst, done
 
 
+def cpu_proxy_baseline(batch, out, seconds: float, threads: int):
+    """Time the GKL-class CPU proxy (oracle/simd_proxy.c: FP32 SIMD lanes over a haplotype's reads, FP64
+    recompute under 1e-28) on whole regions of the workload, in random order, until ~`seconds`."""
+    from oracle import oracle
+
+    rng = np.random.default_rng(4321)
+    regions = rng.permutation(batch.n_regions)
+    done, cells, t_used, worst, rescued = 0, 0, 0.0, 0.0, 0
+    step = 8
+    while t_used < seconds and done < len(regions):
+        sel = np.sort(regions[done:done + step])
+        idx = np.concatenate([np.arange(batch.region_pair_off[k], batch.region_pair_off[k + 1]) for k in sel])
+        t0 = time.perf_counter()
+        got, n = oracle.simd_proxy_batch(batch, n_threads=threads, pairs=idx)
+        t_used += time.perf_counter() - t0
+        rescued += n
+        cells += batch.cells(idx)
+        fin = np.isfinite(got)
+        if out is not None and fin.any():
+            worst = max(worst, float(np.max(np.abs(out[idx][fin] - got[fin]))))
+        done += len(sel)
+        step = min(step * 2, 256)
+    return dict(value=round(cells / t_used / 1e9, 3), unit="GCUPS", cores=threads, kind="port",
+                sample=f"{done} random whole regions of the timed workload ({cells:.3e} cells), FP32 SIMD "
+                       f"(AVX-512 where the host has it) with FP64 recompute of {rescued} pairs, flush-to-zero, "
+                       f"OpenMP over haplotypes, {t_used:.1f} s",
+                max_abs_dlog10L_vs_gpu=worst)
+
+
 def fwd_bwd_leg(args):
     """Forward + backward GCUPS on a batched_uneven batch and forward GCUPS on the same pairs."""
     import torch
@@ -659,11 +688,13 @@ def main() -> int:
     precision = None
     if args.precision_regions > 0:
         precision = precision_leg(args, min(16, os.cpu_count() or 1))
-    cpu, worst, n_checked = None, None, 0
+    cpu, worst, n_checked, proxy = None, None, 0, None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu, worst, n_checked = cpu_baseline(batch, out, args.cpu_seconds, threads)
         log(f"cpu baseline {cpu['value']:.3f} GCUPS on {threads} threads; max |dlog10L| on {n_checked} pairs = {worst:.2e}")
+        proxy = cpu_proxy_baseline(batch, out, args.cpu_seconds, threads)
+        log(f"cpu GKL-class proxy: {proxy}")
         if bwd is not None:
             bwd.update(cpu_grad_baseline(bwd_ctx, threads))
 
@@ -705,6 +736,7 @@ def main() -> int:
                      "kernel_ms": round(st.fp32_kernel_ms, 3),
                      "flop_per_cell": FLOP_PER_CELL},
         "cpu_baseline": cpu,
+        "cpu_gkl_class_proxy": proxy,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -74,6 +74,10 @@ def lib() -> ctypes.CDLL:
             L.oracle_sw_align.argtypes = [_u8p, ctypes.c_int, _u8p, ctypes.c_int] + [ctypes.c_int] * 5 + [
                 ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _i32p, _i32p]
             L.oracle_sw_align.restype = ctypes.c_int
+            L.oracle_simd_proxy_batch.argtypes = [ctypes.c_int64, _i32p, _i32p, _u8p, _u8p, _u8p, _u8p, _u8p,
+                                                  _i64p, _i32p, _u8p, _i64p, _i32p, ctypes.c_int32, ctypes.c_int,
+                                                  ctypes.c_int, _dp]
+            L.oracle_simd_proxy_batch.restype = ctypes.c_long
             L.oracle_log10.argtypes = pair + [ctypes.c_int, _dp]
             L.oracle_log10.restype = ctypes.c_int
             L.oracle_logless_batch.argtypes = [ctypes.c_int64, _i32p, _i32p, _u8p, _u8p, _u8p, _u8p, _u8p,
@@ -269,6 +273,29 @@ def logless_batch(batch, n_threads: int = 1, tristate: bool = True, pairs=None) 
     if st != 0:
         raise OracleError(f"oracle_logless_batch status {st}")
     return out
+
+
+def simd_proxy_batch(batch, n_threads: int = 1, tristate: bool = True, pairs=None):
+    """The GKL-class CPU proxy (simd_proxy.c: FP32 SIMD lanes over reads, FP64 recompute under 1e-28)
+    over a packed batch; returns (log10 L per selected pair, pairs recomputed in FP64)."""
+    pr = np.ascontiguousarray(batch.pair_read, dtype=np.int32)
+    ph = np.ascontiguousarray(batch.pair_hap, dtype=np.int32)
+    if pairs is not None:
+        pairs = np.asarray(pairs)
+        pr, ph = np.ascontiguousarray(pr[pairs]), np.ascontiguousarray(ph[pairs])
+    out = np.empty(len(pr), dtype=np.float64)
+    ro = np.ascontiguousarray(batch.read_off, dtype=np.int64)
+    rl = np.ascontiguousarray(batch.read_len, dtype=np.int32)
+    ho = np.ascontiguousarray(batch.hap_off, dtype=np.int64)
+    hl = np.ascontiguousarray(batch.hap_len, dtype=np.int32)
+    n = lib().oracle_simd_proxy_batch(len(pr), _p(pr, _i32p), _p(ph, _i32p), _p(batch.read_bases),
+                                      _p(batch.read_quals), _p(batch.read_ins), _p(batch.read_del),
+                                      _p(batch.read_gcp), _p(ro, _i64p), _p(rl, _i32p), _p(batch.hap_bases),
+                                      _p(ho, _i64p), _p(hl, _i32p), len(hl), int(tristate), int(n_threads),
+                                      _p(out, _dp))
+    if n < 0:
+        raise OracleError("oracle_simd_proxy_batch: out of memory")
+    return out, int(n)
 
 
 def _f64(a) -> np.ndarray:
