@@ -10,7 +10,8 @@ product path never does.
   tables (PairHMMModel.java:71-117, 373-418; QualityUtils.java:43-57; MathUtils.java:406-480),
   plus an FP64 adjoint of the recurrence for continuous Phred parameters, and the partially
   determined ``LoglessPDPairHMM`` (LoglessPDPairHMM.java:34-194); ``sw_oracle.c`` restates
-  ``SmithWatermanJavaAligner`` (SmithWatermanJavaAligner.java:104-377).
+  ``SmithWatermanJavaAligner`` (SmithWatermanJavaAligner.java:104-377); ``simd_proxy.c`` is not a
+  restatement but a GKL-class FP32 SIMD CPU PairHMM that only the bench's CPU leg times.
 * ``torch_log10_likelihood`` is an independent anti-diagonal restatement of the same continuous
   model in torch FP64, used with ``torch.autograd.gradcheck`` to pin the C adjoint.
 """
