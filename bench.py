@@ -59,6 +59,20 @@ def cpu_baseline(batch, out, seconds: float, threads: int):
                        f"(oracle/pairhmm_oracle.c), OpenMP over pairs, {t_used:.1f} s"), worst, done
 
 
+def host_cpu() -> str:
+    """The host CPU model and logical CPU count (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def cpu_proxy_baseline(batch, out, seconds: float, threads: int):
     """Time the GKL-class CPU proxy (oracle/simd_proxy.c: FP32 SIMD lanes over a haplotype's reads, FP64
     recompute under 1e-28) on whole regions of the workload, in random order, until ~`seconds`."""
@@ -693,8 +707,11 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         cpu, worst, n_checked = cpu_baseline(batch, out, args.cpu_seconds, threads)
         log(f"cpu baseline {cpu['value']:.3f} GCUPS on {threads} threads; max |dlog10L| on {n_checked} pairs = {worst:.2e}")
+        one, _, _ = cpu_baseline(batch, None, args.cpu_seconds / 3, 1)
+        cpu["value_1_thread"] = round(one["value"], 4)
+        cpu["host"] = host_cpu()
         proxy = cpu_proxy_baseline(batch, out, args.cpu_seconds, threads)
-        log(f"cpu GKL-class proxy: {proxy}")
+        log(f"cpu GKL-class proxy: {proxy}; FP64 port on 1 thread {one['value']:.3f} GCUPS; host {cpu['host']}")
         if bwd is not None:
             bwd.update(cpu_grad_baseline(bwd_ctx, threads))
 
