@@ -12,8 +12,12 @@
  *   done()                                         :164       -> pairhmm_done
  * A JNI shim maps those four Java methods onto these functions one to one.
  *
- * The later blocks are new entry points with no counterpart in the reference: a device-resident
- * batch API (one persistent launch over many active regions) and the backward (adjoint) API.
+ * Later blocks: new entry points with no counterpart in the reference (a device-resident batch API,
+ * one persistent launch over many active regions; streaming; the backward (adjoint) API; tuning;
+ * DRAGstr; read-quality preparation and likelihood post-processing on the device), and two more
+ * drop-in boundaries of GATK's native libraries: GKL's IntelPDHMM (partially determined PairHMM,
+ * VectorLoglessPairPDHMM.java:115) and GKL's SWAlignerNativeBinding (Smith-Waterman,
+ * SWNativeAlignerWrapper.java:46).
  *
  * All functions return a pairhmm_status code; pairhmm_last_error() describes the last failure of
  * the calling thread.  Pointers are borrowed for the duration of the call only.
