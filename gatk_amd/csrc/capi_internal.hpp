@@ -109,6 +109,9 @@ inline int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *
 struct QueueSlice {
     int64_t begin = 0, count = 0;
     int max_hap_len = 1;
+    // The slice's last n_direct queue entries go to the FP64 kernel without an FP32 pass (pairs whose
+    // read predicts an FP32 underflow); they also open the slice's rescue list.
+    int64_t n_direct = 0;
 };
 // Device counters of a batch pass, per queue slice k: [k] FP32 queue head, [16 + k] pairs the slice
 // sent to the FP64 rescue (its rescue list sits at the slice's own offset in the rescue array),
@@ -125,10 +128,23 @@ inline uint32_t rescued_total(const uint32_t *host_counters) {
     return n;
 }
 
+// Expected log10 emission of a read under its own base qualities, summed over its bases (per base
+// (1-e) log10(1-e) + e log10(e/3)).  A pair whose read scores below kDirectFp64Score is sent to the
+// FP64 kernel without an FP32 pass: its FP32 sum would fall under the rescue threshold (log10 L near
+// -64) with near certainty, and the FP64 result is the one it would get after the rescue anyway.
+constexpr double kDirectFp64Score = -90.0;
+
 // Validates a host batch descriptor and builds its work queue and slices.  Fills cells (sum of
-// R*H) and the longest haplotype.
+// R*H) and the longest haplotype.  With route_direct, pairs whose read scores below
+// kDirectFp64Score are placed at the end of their slice (QueueSlice::n_direct).
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_hap_len, std::vector<QueueSlice> &slices);
+                           int &max_hap_len, std::vector<QueueSlice> &slices, bool route_direct = false);
+
+// Counter values at the start of a forward pass, and the initial rescue list, for slices built with
+// route_direct: each slice's rescue count starts at its direct FP64 pairs, which sit at the head of
+// its rescue list.
+void direct_rescue_init(const std::vector<QueueSlice> &slices, const std::vector<uint32_t> &order,
+                        std::vector<uint32_t> &counter_init, std::vector<uint32_t> &rescue_init);
 
 }  // namespace capi
 }  // namespace gatk_pairhmm

@@ -28,6 +28,7 @@ struct pairhmm_batch {
     uint32_t *d_order = nullptr;    // work queue: pair indices by descending cost
     uint32_t *d_rescue = nullptr;   // rescue list (n_pairs capacity)
     uint32_t *d_counters = nullptr; // kCounterSlots: queue heads and rescue counters
+    uint32_t *d_counter_init = nullptr; // their values at the start of a pass
     hipGraphExec_t graph = nullptr;
     std::vector<QueueSlice> slices; // FP32 and FP64 launches over the queue, one per LDS class run
     double *d_f64 = nullptr;        // FP64 boundary planes of the batch's FP64 launches
@@ -228,8 +229,20 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
 namespace gatk_pairhmm {
 namespace capi {
 
+void direct_rescue_init(const std::vector<QueueSlice> &slices, const std::vector<uint32_t> &order,
+                        std::vector<uint32_t> &counter_init, std::vector<uint32_t> &rescue_init) {
+    counter_init.assign(kCounterSlots, 0u);
+    rescue_init.assign(order.size(), 0u);
+    for (size_t k = 0; k < slices.size(); ++k) {
+        const QueueSlice &q = slices[k];
+        *slice_rescue_count(counter_init.data(), k) = static_cast<uint32_t>(q.n_direct);
+        for (int64_t i = 0; i < q.n_direct; ++i)
+            rescue_init[q.begin + i] = order[q.begin + q.count - q.n_direct + i];
+    }
+}
+
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_h, std::vector<QueueSlice> &slices) {
+                           int &max_h, std::vector<QueueSlice> &slices, bool route_direct) {
     if (d->n_reads < 0 || d->n_haps <= 0 || d->n_pairs < 0 || d->read_bytes < 0 || d->hap_bytes <= 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
     if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "batch exceeds 2^32 pairs");
@@ -322,24 +335,49 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         }
     }
     const int n_runs = static_cast<int>(run_class.size());
-    std::vector<int64_t> run_n(static_cast<size_t>(n_runs), 0);
+    std::vector<int64_t> run_n(static_cast<size_t>(n_runs), 0), run_direct(static_cast<size_t>(n_runs), 0);
     std::vector<int> run_h(static_cast<size_t>(n_runs), 1);
     for (int k = 0; k <= top; ++k) {
         run_n[run_of[k]] += n_in[k];
         if (n_in[k] > 0) run_h[run_of[k]] = std::max(run_h[run_of[k]], h_in[k]);
     }
-    // Largest class first: runs in descending order of their class.
-    std::vector<int64_t> pos(static_cast<size_t>(n_runs), 0);
+    std::vector<uint8_t> direct;
+    if (route_direct) {
+        double score_of_q[256];
+        for (int q = 0; q < 256; ++q) {
+            const double e = std::pow(10.0, -q / 10.0);
+            score_of_q[q] = e < 1.0 ? (1.0 - e) * std::log10(1.0 - e) + e * std::log10(e / 3.0) : -1.0;
+        }
+        std::vector<uint8_t> read_direct(static_cast<size_t>(d->n_reads), 0);
+        for (int64_t r = 0; r < d->n_reads; ++r) {
+            double sc = 0.0;
+            const uint8_t *q = d->read_quals + d->read_off[r];
+            for (int32_t i = 0; i < d->read_len[r]; ++i) sc += score_of_q[q[i]];
+            read_direct[r] = sc < kDirectFp64Score;
+        }
+        direct.resize(static_cast<size_t>(np));
+        for (int64_t p = 0; p < np; ++p) {
+            direct[p] = read_direct[d->pair_read[p]];
+            if (direct[p]) run_direct[run_of[lds_class_index(d->hap_len[d->pair_hap[p]])]]++;
+        }
+    }
+    // Largest class first: runs in descending order of their class; within a run the FP32 queue by
+    // descending cost, then the direct FP64 pairs by descending cost.
+    std::vector<int64_t> pos(static_cast<size_t>(n_runs), 0), pos_direct(static_cast<size_t>(n_runs), 0);
     int64_t acc = 0;
     for (int r = n_runs - 1; r >= 0; --r) {
         pos[r] = acc;
-        if (run_n[r] > 0) slices.push_back(QueueSlice{acc, run_n[r], run_h[r]});
+        pos_direct[r] = acc + run_n[r] - run_direct[r];
+        if (run_n[r] > 0) slices.push_back(QueueSlice{acc, run_n[r], run_h[r], run_direct[r]});
         acc += run_n[r];
     }
     for (int64_t i = 0; i < np; ++i) {
         const uint32_t p = by_cost[i];
         const int r = run_of[lds_class_index(d->hap_len[d->pair_hap[p]])];
-        order[pos[r]++] = p;
+        if (!direct.empty() && direct[p])
+            order[pos_direct[r]++] = p;
+        else
+            order[pos[r]++] = p;
     }
     return PAIRHMM_OK;
 }
@@ -358,10 +396,12 @@ hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &bat
         const QueueSlice &q = slices[k];
         const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block);
         if (sh.waves_per_block == 0) return hipErrorInvalidValue;
+        const int64_t n32 = q.count - q.n_direct;
+        if (n32 == 0) continue;
         RescueSink rs{rescue_list + q.begin, slice_rescue_count(counters, k), kFp32RescueThreshold};
-        WorkSource w{order + q.begin, static_cast<uint32_t>(q.count), nullptr, slice_head(counters, k)};
+        WorkSource w{order + q.begin, static_cast<uint32_t>(n32), nullptr, slice_head(counters, k)};
         const hipError_t e = launch_forward(false, batch, tab, w, rs, q.max_hap_len, sh,
-                                            persistent_grid(ctx, sh, q.count), s);
+                                            persistent_grid(ctx, sh, n32), s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -415,9 +455,12 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices);
+    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true);
     if (vst != PAIRHMM_OK) return vst;
     const int64_t np = d->n_pairs;
+    // Counter values at the start of every pass (the rescue list's head is copied once, below).
+    std::vector<uint32_t> counter_init, rescue_init;
+    direct_rescue_init(slices, order, counter_init, rescue_init);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
@@ -428,7 +471,8 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     const size_t o_rb = take(rb), o_rq = take(rb), o_ri = take(rb), o_rd = take(rb), o_rg = take(rb), o_hb = take(hb),
                  o_roff = take(8 * d->n_reads), o_rlen = take(4 * d->n_reads), o_hoff = take(8 * d->n_haps),
                  o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np),
-                 o_out = take(8 * np), o_res = take(4 * np), o_cnt = take(4 * kCounterSlots);
+                 o_out = take(8 * np), o_res = take(4 * np), o_cnt = take(4 * kCounterSlots),
+                 o_cinit = take(4 * kCounterSlots);
     HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
     pairhmm_batch *bt = new (std::nothrow) pairhmm_batch();
     if (!bt) return fail(PAIRHMM_ERR_OOM, "batch allocation failed");
@@ -450,7 +494,8 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
                   {o_roff, d->read_off, 8 * (size_t)d->n_reads}, {o_rlen, d->read_len, 4 * (size_t)d->n_reads},
                   {o_hoff, d->hap_off, 8 * (size_t)d->n_haps},   {o_hlen, d->hap_len, 4 * (size_t)d->n_haps},
                   {o_pr, d->pair_read, 4 * (size_t)np},          {o_ph, d->pair_hap, 4 * (size_t)np},
-                  {o_ord, order.data(), 4 * (size_t)np}};
+                  {o_ord, order.data(), 4 * (size_t)np},         {o_res, rescue_init.data(), 4 * (size_t)np},
+                  {o_cinit, counter_init.data(), 4 * (size_t)kCounterSlots}};
     for (const Cp &c : copies) {
         if (c.n == 0) continue;
         hipError_t e = hipMemcpy(m + c.o, c.src, c.n, hipMemcpyHostToDevice);
@@ -477,6 +522,7 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     bt->d_order = reinterpret_cast<uint32_t *>(m + o_ord);
     bt->d_rescue = reinterpret_cast<uint32_t *>(m + o_res);
     bt->d_counters = reinterpret_cast<uint32_t *>(m + o_cnt);
+    bt->d_counter_init = reinterpret_cast<uint32_t *>(m + o_cinit);
     bt->slices = std::move(slices);
     bt->f64_bytes = fp64_slices_plane_bytes(ctx, bt->slices);
     if (bt->f64_bytes > 0 && hipMalloc(reinterpret_cast<void **>(&bt->d_f64), bt->f64_bytes) != hipSuccess) {
@@ -500,7 +546,7 @@ namespace {
 hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
     pairhmm_context *ctx = bt->ctx;
     DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
-    hipError_t e = launch_zero_counters(bt->d_counters, kCounterSlots, s);
+    hipError_t e = launch_copy_counters(bt->d_counters, bt->d_counter_init, kCounterSlots, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[0], s);
     e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s);

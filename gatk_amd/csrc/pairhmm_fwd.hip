@@ -427,6 +427,10 @@ __global__ void zero_counters_kernel(uint32_t *c, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
 }
 
+__global__ void copy_counters_kernel(uint32_t *c, const uint32_t *init, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = init[i];
+}
+
 }  // namespace
 
 int lds_columns(int max_hap_len) {
@@ -481,6 +485,11 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
 
 hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream) {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_counters(uint32_t *counters, const uint32_t *init, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(copy_counters_kernel, dim3(1), dim3(64), 0, stream, counters, init, n);
     return hipGetLastError();
 }
 

@@ -105,6 +105,8 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
 
 // Zeroes n counters with a kernel (graph-capturable; no memset node).
 hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t stream);
+// Sets n counters from a device array with a kernel (graph-capturable).
+hipError_t launch_copy_counters(uint32_t *counters, const uint32_t *init, int n, hipStream_t stream);
 
 // ---- forward + backward (adjoint) over continuous Phred parameters --------------------------
 

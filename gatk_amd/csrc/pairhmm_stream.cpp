@@ -97,8 +97,10 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    st = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices);
+    st = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true);
     if (st != PAIRHMM_OK) return st;
+    std::vector<uint32_t> counter_init, rescue_init;
+    direct_rescue_init(slices, order, counter_init, rescue_init);
     const int64_t np = d->n_pairs;
     if (np == 0) return PAIRHMM_OK;
     const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block);
@@ -115,9 +117,10 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     const size_t rb = static_cast<size_t>(d->read_bytes), hb = static_cast<size_t>(d->hap_bytes);
     const size_t o_rb = take(rb), o_rq = take(rb), o_ri = take(rb), o_rd = take(rb), o_rg = take(rb), o_hb = take(hb),
                  o_roff = take(8 * d->n_reads), o_rlen = take(4 * d->n_reads), o_hoff = take(8 * d->n_haps),
-                 o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np);
+                 o_hlen = take(4 * d->n_haps), o_pr = take(4 * np), o_ph = take(4 * np), o_ord = take(4 * np),
+                 o_res = take(4 * np), o_cinit = take(4 * kCounterSlots);
     const size_t in_bytes = off;
-    const size_t o_out = take(8 * np), o_cnt = take(4 * kCounterSlots), o_res = take(4 * np);
+    const size_t o_out = take(8 * np), o_cnt = take(4 * kCounterSlots);
     const size_t f64_bytes = fp64_slices_plane_bytes(ctx, slices);
     if (sl.hin.reserve(in_bytes) != hipSuccess || sl.hout.reserve(8 * np + 4 * kCounterSlots) != hipSuccess ||
         sl.dmem.reserve(off) != hipSuccess || sl.f64.reserve(f64_bytes) != hipSuccess)
@@ -132,7 +135,8 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
                   {o_roff, d->read_off, 8 * (size_t)d->n_reads}, {o_rlen, d->read_len, 4 * (size_t)d->n_reads},
                   {o_hoff, d->hap_off, 8 * (size_t)d->n_haps},   {o_hlen, d->hap_len, 4 * (size_t)d->n_haps},
                   {o_pr, d->pair_read, 4 * (size_t)np},          {o_ph, d->pair_hap, 4 * (size_t)np},
-                  {o_ord, order.data(), 4 * (size_t)np}};
+                  {o_ord, order.data(), 4 * (size_t)np},         {o_res, rescue_init.data(), 4 * (size_t)np},
+                  {o_cinit, counter_init.data(), 4 * (size_t)kCounterSlots}};
     for (const Cp &c : copies)
         if (c.n) std::memcpy(h + c.o, c.src, c.n);
     uint8_t *m = static_cast<uint8_t *>(sl.dmem.ptr);
@@ -160,7 +164,8 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     DeviceTables tab{ctx->d_qual_err, ctx->d_mm_prob};
     hipStream_t cs = ctx->stream;
     HIP_TRY(hipStreamWaitEvent(cs, sl.h2d_done, 0), PAIRHMM_ERR_KERNEL);
-    HIP_TRY(launch_zero_counters(cnt, kCounterSlots, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_copy_counters(cnt, reinterpret_cast<const uint32_t *>(m + o_cinit), kCounterSlots, cs),
+            PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_fp64_slices(ctx, v, tab, res, slices, cnt, true, static_cast<double *>(sl.f64.ptr), sl.f64.cap, cs),
             PAIRHMM_ERR_KERNEL);

@@ -215,10 +215,21 @@ def test_rescued_pairs_equal_fp64_kernel(engine):
     out64 = np.empty(b.n_pairs)
     dev.forward(mode=_native.RUN_FP64, out=out64)
     dev.free()
-    rescued = out + 120 * np.log10(2.0) < -28.0  # FP32 sum L * 2^120 below 1e-28
+    # FP64 pairs: those the FP32 pass flagged (FP32 sum L * 2^120 below 1e-28) and those the batch sent
+    # there directly because their read's expected log10 emission is below -90 (the upload's routing).
+    rescued = (out + 120 * np.log10(2.0) < -28.0) | _predicted_direct(b)
     assert st.n_rescued == int(rescued.sum()) and 0 < st.n_rescued < b.n_pairs
     np.testing.assert_array_equal(out[rescued], out64[rescued])
     assert np.max(np.abs(out[~rescued] - out64[~rescued])) <= FP32_TOL
+
+
+def _predicted_direct(b):
+    """Pairs whose read scores sum((1-e) log10(1-e) + e log10(e/3)) < -90 over its base qualities."""
+    score_of_q = [(1 - e) * math.log10(1 - e) + e * math.log10(e / 3) if e < 1 else -1.0
+                  for e in (10.0 ** (-q / 10.0) for q in range(256))]
+    read_score = np.array([sum(score_of_q[q] for q in b.read_quals[o:o + n].tolist())  # read order
+                           for o, n in zip(b.read_off.tolist(), b.read_len.tolist())])
+    return read_score[b.pair_read] < -90.0
 
 
 def test_concurrent_contexts():
