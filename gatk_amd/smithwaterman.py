@@ -56,8 +56,12 @@ def last_index_of(reference: bytes, query: bytes) -> int:
     return bytes(reference).rfind(bytes(query))
 
 
-def _cigar_cap(ref_len: int, alt_len: int) -> int:
-    return 12 * (ref_len + alt_len + 4) + 1
+def _cigar_cap(ref_len, alt_len):
+    """Bytes that always hold an alignment's CIGAR text and its NUL: at most ref + alt + 4 elements
+    (every element but the clips consumes a base), each at most the longer sequence's digits + 1."""
+    longest = np.maximum(ref_len, alt_len)
+    digits = 1 + (longest >= 10).astype(np.int32) + (longest >= 100) + (longest >= 1000) + (longest >= 10000)
+    return (np.asarray(ref_len) + alt_len + 4) * (digits + 1) + 1
 
 
 class HipSmithWatermanBinding:
@@ -77,7 +81,7 @@ class HipSmithWatermanBinding:
 
     def align(self, ref, alt, parameters: SWParameters, strategy: SWOverhangStrategy) -> Tuple[str, int]:
         r, a = np.ascontiguousarray(_as_u8(ref)), np.ascontiguousarray(_as_u8(alt))
-        cap = _cigar_cap(len(r), len(a))
+        cap = int(_cigar_cap(len(r), len(a)))
         buf = ctypes.create_string_buffer(cap)
         off = ctypes.c_int32()
         p = parameters.native()
@@ -99,9 +103,9 @@ class HipSmithWatermanBinding:
         ao = np.concatenate([[0], np.cumsum(al, dtype=np.int64)[:-1]]).astype(np.int64)
         rb = np.frombuffer(b"".join(bytes(x) for x in refs), np.uint8).copy()
         ab = np.frombuffer(b"".join(bytes(x) for x in alts), np.uint8).copy()
-        caps = np.array([_cigar_cap(int(x), int(y)) for x, y in zip(rl, al)], np.int32)
+        caps = _cigar_cap(rl, al).astype(np.int32)
         coff = np.concatenate([[0], np.cumsum(caps, dtype=np.int64)[:-1]]).astype(np.int64)
-        cig = np.zeros(int(caps.sum()), np.uint8)
+        cig = np.empty(int(caps.sum()), np.uint8)  # each record is NUL-terminated by the library
         offs = np.zeros(n, np.int32)
         p = parameters.native()
         st = self._lib.pairhmm_sw_align_batch(self._ctx, n, rb.ctypes.data, ro.ctypes.data, rl.ctypes.data,
@@ -109,11 +113,8 @@ class HipSmithWatermanBinding:
                                               int(strategy), cig.ctypes.data, coff.ctypes.data, caps.ctypes.data,
                                               offs.ctypes.data)
         _raise_for(st, "align_batch")
-        out = []
-        for k in range(n):
-            s = cig[coff[k]:coff[k] + caps[k]].tobytes()
-            out.append((s[:s.index(0)].decode("ascii"), int(offs[k])))
-        return out
+        base = cig.ctypes.data
+        return [(ctypes.string_at(base + o).decode("ascii"), f) for o, f in zip(coff.tolist(), offs.tolist())]
 
     def close(self) -> None:
         if self._lib is not None and self._ctx:
