@@ -41,11 +41,27 @@ __constant__ double kDynVar[40] = {
 constexpr int kPpWaves = 2;             // waves per workgroup, each over 64 consecutive reads
 constexpr int kQualStage = 10240;       // bytes of HBM qualities a wave stages in LDS
 constexpr int kRowStage = 512;          // likelihoods a wave stages in LDS (64 reads x 8 haplotypes)
+constexpr int kQualBatch = 10;          // quality words per lane in flight while staging
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sums of the table's mean and variance over a read's qualities, in read order (the order the
+// reference adds them).  Unrolled so the reads of later bases issue ahead of the ordered FP64 sums.
+__device__ __forceinline__ void quality_sums(const uint8_t *src, int L, const double2 *tab, double &s_mean,
+                                             double &s_var) {
+#pragma clang fp contract(off)
+#pragma unroll 8
+    for (int i = 0; i < L; ++i) {
+        const int bq = src[i];
+        const int e = bq <= 1 ? 0 : (bq < 40 ? bq : 40) - 1;
+        const double2 mv = tab[e];
+        s_mean += mv.x;
+        s_var += mv.y;
+    }
 }
 
 __global__ __launch_bounds__(kPpWaves * 64) void postprocess_kernel(PostprocessArgs p) {
@@ -69,10 +85,73 @@ __global__ __launch_bounds__(kPpWaves * 64) void postprocess_kernel(PostprocessA
 
     const bool stage_rows = static_cast<int64_t>(nr) * A <= kRowStage;
     double *const grow = p.likelihoods + r0 * A;
+    const int L = has ? p.read_len[r] : 0;
+    const uint8_t *q = p.base_quals + (has && p.dynamic ? p.read_off[r] : 0);
+    // Every HBM load of the wave is issued before the first wait: the row loads go to registers
+    // first (nr * A <= kRowStage = 8 * 64, so eight per lane), then the qualities are staged, then
+    // the rows are stored to LDS; one wave barrier covers both.
+    double rt[kRowStage / 64];
     if (stage_rows) {
-        for (int e = lane; e < nr * A; e += 64) rstage[w][e] = grow[e];
-        wave_lds_sync();
+#pragma unroll
+        for (int u = 0; u < kRowStage / 64; ++u) {
+            const int e = lane + 64 * u;
+            rt[u] = e < nr * A ? grow[e] : 0.0;
+        }
     }
+    // The wave's quality span [lo, hi) in global addresses; staged when it fits, read in place
+    // otherwise.
+    bool stage_q = false;
+    uint64_t blo = 0;
+    uint8_t *const qb = reinterpret_cast<uint8_t *>(qstage[w]);
+    if (p.dynamic) {
+        uint64_t lo = has ? reinterpret_cast<uintptr_t>(q) : ~0ull, hi = has ? lo + L : 0ull;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t olo = __shfl_xor(static_cast<unsigned long long>(lo), off, 64);
+            const uint64_t ohi = __shfl_xor(static_cast<unsigned long long>(hi), off, 64);
+            lo = olo < lo ? olo : lo;
+            hi = ohi > hi ? ohi : hi;
+        }
+        blo = lo & ~3ull;
+        stage_q = hi > lo && hi - blo <= static_cast<uint64_t>(kQualStage);
+        if (stage_q) {
+            const uint64_t wlo = (lo + 3) & ~3ull, whi = hi & ~3ull;
+            // Derived from the plane's pointer (not from the integer address) so the loads stay global.
+            const uint8_t *gb = p.base_quals + (blo - reinterpret_cast<uintptr_t>(p.base_quals));
+            if (wlo < whi) {
+                const __attribute__((address_space(1))) uint32_t *gw =
+                    (const __attribute__((address_space(1))) uint32_t *)(gb + (wlo - blo));
+                const int nw = static_cast<int>((whi - wlo) / 4), w0 = static_cast<int>((wlo - blo) / 4);
+                // Batches of kQualBatch loads per lane in flight before their stores.
+                for (int k0 = lane; k0 < nw; k0 += 64 * kQualBatch) {
+                    uint32_t t[kQualBatch];
+#pragma unroll
+                    for (int u = 0; u < kQualBatch; ++u) {
+                        const int k = k0 + 64 * u;
+                        t[u] = k < nw ? gw[k] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kQualBatch; ++u) {
+                        const int k = k0 + 64 * u;
+                        if (k < nw) qstage[w][w0 + k] = t[u];
+                    }
+                }
+                const int head = static_cast<int>(wlo - lo), tail = static_cast<int>(hi - whi);
+                if (lane < head) qb[lo - blo + lane] = gb[lo - blo + lane];
+                if (lane < tail) qb[whi - blo + lane] = gb[whi - blo + lane];
+            } else {
+                for (int k = lane; k < static_cast<int>(hi - lo); k += 64) qb[lo - blo + k] = gb[lo - blo + k];
+            }
+        }
+    }
+    if (stage_rows) {
+#pragma unroll
+        for (int u = 0; u < kRowStage / 64; ++u) {
+            const int e = lane + 64 * u;
+            if (e < nr * A) rstage[w][e] = rt[u];
+        }
+    }
+    if (stage_rows || stage_q) wave_lds_sync();
     double *v = !has ? rstage[w] : stage_rows ? &rstage[w][lane * A] : p.likelihoods + r * A;
 
     if (has && p.normalize && A >= 2) {
@@ -88,49 +167,16 @@ __global__ __launch_bounds__(kPpWaves * 64) void postprocess_kernel(PostprocessA
         for (int a = 0; a < A; ++a)
             if (v[a] < worst) v[a] = worst;
     }
-    const int L = has ? p.read_len[r] : 0;
     const double errors = ceil(static_cast<double>(L) * p.max_error_per_base);
     double t = (p.dynamic ? errors : fmin(2.0, errors)) * -4.0;
     if (p.dynamic) {
-        // The wave's quality span [lo, hi) in global addresses; staged when it fits, read in place
-        // otherwise.  Sums run in read order, as the reference adds them.
-        const uint8_t *q = p.base_quals + (has ? p.read_off[r] : 0);
-        uint64_t lo = has ? reinterpret_cast<uintptr_t>(q) : ~0ull, hi = has ? lo + L : 0ull;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t olo = __shfl_xor(static_cast<unsigned long long>(lo), off, 64);
-            const uint64_t ohi = __shfl_xor(static_cast<unsigned long long>(hi), off, 64);
-            lo = olo < lo ? olo : lo;
-            hi = ohi > hi ? ohi : hi;
-        }
-        const uint64_t blo = lo & ~3ull;
-        const bool stage_q = hi > lo && hi - blo <= static_cast<uint64_t>(kQualStage);
-        const uint8_t *src = q;
-        if (stage_q) {
-            uint8_t *qb = reinterpret_cast<uint8_t *>(qstage[w]);
-            const uint64_t wlo = (lo + 3) & ~3ull, whi = hi & ~3ull;
-            const uint8_t *gb = reinterpret_cast<const uint8_t *>(blo);
-            if (wlo < whi) {
-                const uint32_t *gw = reinterpret_cast<const uint32_t *>(wlo);
-                const int nw = static_cast<int>((whi - wlo) / 4), w0 = static_cast<int>((wlo - blo) / 4);
-                for (int k = lane; k < nw; k += 64) qstage[w][w0 + k] = gw[k];
-                const int head = static_cast<int>(wlo - lo), tail = static_cast<int>(hi - whi);
-                if (lane < head) qb[lo - blo + lane] = gb[lo - blo + lane];
-                if (lane < tail) qb[whi - blo + lane] = gb[whi - blo + lane];
-            } else {
-                for (int k = lane; k < static_cast<int>(hi - lo); k += 64) qb[lo - blo + k] = gb[lo - blo + k];
-            }
-            wave_lds_sync();
-            src = qb + (reinterpret_cast<uintptr_t>(q) - blo);
-        }
+        // Sums in read order, as the reference adds them; two calls so each loop sees one address
+        // space (LDS or global) and its reads pipeline.
         double s_mean = 0.0, s_var = 0.0;
-        for (int i = 0; i < L; ++i) {
-            const int bq = src[i];
-            const int e = bq <= 1 ? 0 : (bq < 40 ? bq : 40) - 1;
-            const double2 mv = tab[e];
-            s_mean += mv.x;
-            s_var += mv.y;
-        }
+        if (stage_q)
+            quality_sums(qb + (reinterpret_cast<uintptr_t>(q) - blo), L, tab, s_mean, s_var);
+        else
+            quality_sums(q, L, tab, s_mean, s_var);
         const double d = (s_mean + p.scale * sqrt(s_var)) * -0.1;
         if (d < t) t = d;
     }
