@@ -140,6 +140,12 @@ constexpr double kDirectFp64Score = -90.0;
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
                            int &max_hap_len, std::vector<QueueSlice> &slices, bool route_direct = false);
 
+// True when a read's expected log10 emission along a perfect alignment,
+// sum_i (1-e_i) log10(1-e_i) + e_i log10(e_i/3) over its base-quality error rates (each term rounded
+// to 2^-40 and summed exactly in integers), is below kDirectFp64Score: its pairs would underflow the
+// FP32 pass, so they go to the FP64 kernel directly.
+bool read_routes_direct(const uint8_t *quals, int32_t len);
+
 // Counter values at the start of a forward pass, and the initial rescue list, for slices built with
 // route_direct: each slice's rescue count starts at its direct FP64 pairs, which sit at the head of
 // its rescue list.

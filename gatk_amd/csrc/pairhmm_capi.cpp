@@ -119,6 +119,15 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     }
     const int64_t n_pairs = static_cast<int64_t>(n_reads) * n_haps;
     if (n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "too many pairs in one call");
+    const bool fp64_all = ctx->use_double;
+    // Reads whose qualities predict an FP32 underflow: their pairs start on the rescue list and skip
+    // the FP32 pass (the batch and stream APIs route them the same way).
+    std::vector<uint8_t> direct(static_cast<size_t>(n_reads), 0);
+    int64_t n_direct_reads = 0;
+    if (!fp64_all)
+        for (int32_t r = 0; r < n_reads; ++r)
+            n_direct_reads += direct[r] = read_routes_direct(reads[r].read_quals, reads[r].length);
+    const int64_t n_direct = n_direct_reads * n_haps, n32 = n_pairs - n_direct;
 
     // Staging layout (host pinned and device identical): 5 read planes, hap plane, offsets, lengths.
     size_t off = 0;
@@ -130,10 +139,13 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     const size_t o_rb = take(read_bytes), o_rq = take(read_bytes), o_ri = take(read_bytes), o_rd = take(read_bytes),
                  o_rg = take(read_bytes), o_hb = take(hap_bytes), o_roff = take(sizeof(int64_t) * n_reads),
                  o_rlen = take(sizeof(int32_t) * n_reads), o_hoff = take(sizeof(int64_t) * n_haps),
-                 o_hlen = take(sizeof(int32_t) * n_haps);
-    const size_t in_bytes = off;
-    const size_t o_out = take(sizeof(double) * n_pairs), o_rescue = take(sizeof(uint32_t) * n_pairs),
-                 o_cnt = take(sizeof(uint32_t) * 4);
+                 o_hlen = take(sizeof(int32_t) * n_haps), o_cnt = take(sizeof(uint32_t) * 4),
+                 o_order = take(sizeof(uint32_t) * (n_direct ? n32 : 0)),
+                 o_rescue = take(sizeof(uint32_t) * n_pairs);
+    // The counters and the head of the rescue list travel with the inputs; the copy covers the
+    // rescue list up to its direct pairs.
+    const size_t in_bytes = o_rescue + sizeof(uint32_t) * n_direct;
+    const size_t o_out = take(sizeof(double) * n_pairs);
     const size_t total = off;
 
     HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
@@ -165,6 +177,17 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
         std::memcpy(hs + o_hb + pos, haps[h].haplotype_bases, haps[h].length);
         pos += haps[h].length;
     }
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(hs + o_cnt);
+    hcnt[0] = hcnt[2] = hcnt[3] = 0;
+    hcnt[1] = static_cast<uint32_t>(n_direct);
+    if (n_direct) {
+        uint32_t *ord = reinterpret_cast<uint32_t *>(hs + o_order), *res = reinterpret_cast<uint32_t *>(hs + o_rescue);
+        for (int32_t r = 0; r < n_reads; ++r)
+            for (int32_t h = 0; h < n_haps; ++h) {
+                const uint32_t p = static_cast<uint32_t>(static_cast<int64_t>(r) * n_haps + h);
+                *(direct[r] ? res++ : ord++) = p;
+            }
+    }
     uint8_t *ds = static_cast<uint8_t *>(ctx->stage_dev.ptr);
     HIP_TRY(hipMemcpyAsync(ds, hs, in_bytes, hipMemcpyHostToDevice, ctx->stream), PAIRHMM_ERR_KERNEL);
 
@@ -187,7 +210,6 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     uint32_t *counters = reinterpret_cast<uint32_t *>(ds + o_cnt);
     uint32_t *rescue = reinterpret_cast<uint32_t *>(ds + o_rescue);
 
-    const bool fp64_all = ctx->use_double;
     const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block);
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
     if (sh64.waves_per_block == 0 || sh32.waves_per_block == 0)
@@ -198,6 +220,9 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
         return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, static_cast<int64_t>(sh.blocks_per_cu) * cus)));
     };
     WorkSource all{nullptr, static_cast<uint32_t>(n_pairs), nullptr, nullptr};
+    const WorkSource fp32_items = n_direct ? WorkSource{reinterpret_cast<const uint32_t *>(ds + o_order),
+                                                        static_cast<uint32_t>(n32), nullptr, nullptr}
+                                           : all;
     RescueSink none{nullptr, nullptr, 0.0};
     // FP64 boundary planes for every wave of the FP64 grid (a grid-stride launch needs them all).
     const int grid64 = grid_for(sh64);
@@ -209,9 +234,11 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     if (fp64_all) {
         HIP_TRY(launch_forward(true, b, tab, all, none, max_h, sh64, grid64, ctx->stream), PAIRHMM_ERR_KERNEL);
     } else {
-        HIP_TRY(launch_zero_counters(counters, 4, ctx->stream), PAIRHMM_ERR_KERNEL);
+        // counters[1] (the rescue count) starts at the direct pairs, copied in with the inputs.
         RescueSink rs{rescue, counters + 1, kFp32RescueThreshold};
-        HIP_TRY(launch_forward(false, b, tab, all, rs, max_h, sh32, grid_for(sh32), ctx->stream), PAIRHMM_ERR_KERNEL);
+        if (n32 > 0)
+            HIP_TRY(launch_forward(false, b, tab, fp32_items, rs, max_h, sh32, grid_for(sh32), ctx->stream),
+                    PAIRHMM_ERR_KERNEL);
         // FP64 recompute of the flagged pairs; the count is read on the device.
         WorkSource again{rescue, 0, counters + 1, nullptr};
         HIP_TRY(launch_forward(true, b, tab, again, none, max_h, sh64, grid64, ctx->stream), PAIRHMM_ERR_KERNEL);
@@ -228,6 +255,23 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
 
 namespace gatk_pairhmm {
 namespace capi {
+
+bool read_routes_direct(const uint8_t *quals, int32_t len) {
+    // Per-quality terms in 2^-40 fixed point (floor(x * 2^40 + 0.5)): integer sums are exact, so the
+    // predicate does not depend on the summation order and the loop carries no FP latency chain.
+    static const std::vector<int64_t> score_of_q = [] {
+        std::vector<int64_t> t(256);
+        for (int q = 0; q < 256; ++q) {
+            const double e = std::pow(10.0, -q / 10.0);
+            const double v = e < 1.0 ? (1.0 - e) * std::log10(1.0 - e) + e * std::log10(e / 3.0) : -1.0;
+            t[q] = static_cast<int64_t>(std::floor(v * 0x1p40 + 0.5));
+        }
+        return t;
+    }();
+    int64_t sc = 0;
+    for (int32_t i = 0; i < len; ++i) sc += score_of_q[quals[i]];
+    return sc < static_cast<int64_t>(kDirectFp64Score * 0x1p40);
+}
 
 void direct_rescue_init(const std::vector<QueueSlice> &slices, const std::vector<uint32_t> &order,
                         std::vector<uint32_t> &counter_init, std::vector<uint32_t> &rescue_init) {
@@ -343,18 +387,9 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
     }
     std::vector<uint8_t> direct;
     if (route_direct) {
-        double score_of_q[256];
-        for (int q = 0; q < 256; ++q) {
-            const double e = std::pow(10.0, -q / 10.0);
-            score_of_q[q] = e < 1.0 ? (1.0 - e) * std::log10(1.0 - e) + e * std::log10(e / 3.0) : -1.0;
-        }
         std::vector<uint8_t> read_direct(static_cast<size_t>(d->n_reads), 0);
-        for (int64_t r = 0; r < d->n_reads; ++r) {
-            double sc = 0.0;
-            const uint8_t *q = d->read_quals + d->read_off[r];
-            for (int32_t i = 0; i < d->read_len[r]; ++i) sc += score_of_q[q[i]];
-            read_direct[r] = sc < kDirectFp64Score;
-        }
+        for (int64_t r = 0; r < d->n_reads; ++r)
+            read_direct[r] = read_routes_direct(d->read_quals + d->read_off[r], d->read_len[r]);
         direct.resize(static_cast<size_t>(np));
         for (int64_t p = 0; p < np; ++p) {
             direct[p] = read_direct[d->pair_read[p]];

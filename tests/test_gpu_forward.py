@@ -223,13 +223,42 @@ def test_rescued_pairs_equal_fp64_kernel(engine):
     assert np.max(np.abs(out[~rescued] - out64[~rescued])) <= FP32_TOL
 
 
+def test_drop_in_routes_like_the_batch(engine):
+    """computeLikelihoods per region on STR-rich regions: pairs whose read predicts an FP32 underflow
+    go straight to the FP64 kernel here too, so they carry the batch path's FP64 values bit for bit;
+    every other pair is within the FP32 tolerance of the batch."""
+    b = synth.workload("precision_stress", n_regions=3, seed=7)
+    dev = engine.upload(b)
+    want = np.empty(b.n_pairs)
+    dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=want)
+    dev.free()
+    binding = _binding(False)
+    got = np.empty(b.n_pairs)
+    try:
+        for k in range(len(b.region_pair_off) - 1):
+            reads, haps = b.region(k)
+            p0, p1 = int(b.region_pair_off[k]), int(b.region_pair_off[k + 1])
+            out = np.empty(p1 - p0)
+            binding.compute_likelihoods([ReadDataHolder(*r) for r in reads], [HaplotypeDataHolder(h) for h in haps],
+                                        out)
+            got[p0:p1] = out
+    finally:
+        binding.done()
+    direct = _predicted_direct(b)
+    assert 0 < int(direct.sum()) < b.n_pairs
+    np.testing.assert_array_equal(got[direct], want[direct])
+    assert np.max(np.abs(got - want)) <= FP32_TOL
+
+
 def _predicted_direct(b):
-    """Pairs whose read scores sum((1-e) log10(1-e) + e log10(e/3)) < -90 over its base qualities."""
-    score_of_q = [(1 - e) * math.log10(1 - e) + e * math.log10(e / 3) if e < 1 else -1.0
-                  for e in (10.0 ** (-q / 10.0) for q in range(256))]
-    read_score = np.array([sum(score_of_q[q] for q in b.read_quals[o:o + n].tolist())  # read order
+    """Pairs whose read scores sum((1-e) log10(1-e) + e log10(e/3)) < -90 over its base qualities,
+    each term rounded to 2^-40 fixed point and summed exactly."""
+    score_of_q = np.array([math.floor(((1 - e) * math.log10(1 - e) + e * math.log10(e / 3) if e < 1 else -1.0)
+                                      * 2.0 ** 40 + 0.5) for e in (10.0 ** (-q / 10.0) for q in range(256))],
+                          dtype=np.int64)
+    read_score = np.array([int(score_of_q[b.read_quals[o:o + n]].sum())
                            for o, n in zip(b.read_off.tolist(), b.read_len.tolist())])
-    return read_score[b.pair_read] < -90.0
+    return read_score[b.pair_read] < -90 * 2 ** 40
 
 
 def test_concurrent_contexts():
