@@ -134,7 +134,9 @@ typedef struct pairhmm_run_stats {
     double total_ms;        /* wall time of the device work (HIP events), excluding D2H        */
     double fp32_kernel_ms;  /* duration of the FP32 forward kernel (eager mode only)           */
     double rescue_kernel_ms;/* duration of the FP64 rescue kernel (eager mode only)            */
-    int64_t n_rescued;      /* pairs recomputed in FP64                                        */
+    int64_t n_rescued;      /* pairs computed by the FP64 kernel: flagged by the FP32 pass, or
+                               sent there directly because their read's qualities predict an
+                               FP32 underflow (expected log10 emission below -90)              */
     int64_t cells;          /* sum over pairs of read length x haplotype length                */
 } pairhmm_run_stats;
 
