@@ -109,22 +109,26 @@ def marshal_planes(dtype: np.dtype, columns):
     one contiguous plane and returns a holder array of ``dtype`` whose pointer fields (in field
     order) address each item's bytes, plus the planes to keep alive for the call."""
     n = len(columns[0])
-    lens = np.fromiter(map(len, columns[0]), dtype=np.int64, count=n)
-    for c in columns[1:]:
-        if not np.array_equal(np.fromiter(map(len, c), dtype=np.int64, count=n), lens):
-            raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
+    if any(len(c) != n for c in columns):
+        raise ValueError("columns differ in item count")
+    items = [x for c in columns for x in c]
+    all_lens = np.fromiter(map(len, items), dtype=np.int64, count=len(items)).reshape(len(columns), n)
+    lens = all_lens[0]
+    if not (all_lens == lens).all():
+        raise ValueError("Read bases and read quals / GOP / GCP arrays aren't the same size")
     offs = np.zeros(n, dtype=np.uint64)
     if n > 1:
         offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
     holders = np.zeros(n, dtype=dtype)
-    planes = []
-    empty = n == 0 or int(lens.sum()) == 0
-    for name, c in zip(dtype.names, columns):
-        plane = np.zeros(1, np.uint8) if empty else _concat_u8(c)
-        planes.append(plane)
-        holders[name] = np.uint64(plane.ctypes.data) + offs
+    total = int(lens.sum())
+    # One concatenation for every column (column k's plane starts at k * total): a single call's
+    # overhead instead of one per column.
+    plane = np.zeros(1, np.uint8) if n == 0 or total == 0 else _concat_u8(items)
+    base = plane.ctypes.data
+    for k, name in enumerate(dtype.names[:len(columns)]):
+        holders[name] = np.uint64(base + k * total) + offs
     holders["length"] = lens
-    return holders, planes
+    return holders, [plane]
 
 
 def marshal_reads(reads):
