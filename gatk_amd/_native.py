@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_normalize_and_filter_likelihoods",
     "pairhmm_sw_align",
     "pairhmm_sw_align_batch",
+    "pairhmm_sw_align_batch_packed",
 )
 
 
@@ -233,6 +234,9 @@ def load_library() -> ctypes.CDLL:
         L.pairhmm_sw_align_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32, vp,
                                              vp, vp, vp]
         L.pairhmm_sw_align_batch.restype = i32
+        L.pairhmm_sw_align_batch_packed.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32,
+                                                    vp, i64, vp, vp]
+        L.pairhmm_sw_align_batch_packed.restype = i32
         _lib = L
         return L
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <string>
 
@@ -34,14 +35,13 @@ int32_t format_cigar(const uint32_t *e, int n, char *out, int32_t cap) {
     return pos;
 }
 
-}  // namespace
-
-extern "C" int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases,
-                                          const int64_t *ref_off, const int32_t *ref_len, const uint8_t *alt_bases,
-                                          const int64_t *alt_off, const int32_t *alt_len,
-                                          const pairhmm_sw_parameters *params, int32_t overhang_strategy,
-                                          char *cigars, const int64_t *cigar_off, const int32_t *cigar_cap,
-                                          int32_t *alignment_offset) {
+// The device side of both batch forms: validation, staging, one launch, results back in the
+// context's pinned staging.  On success *elems / *n_el / *offs point there (pair p's elements start
+// at the sum over q < p of ref_len[q] + alt_len[q] + 4).
+int32_t sw_run_batch(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases, const int64_t *ref_off,
+                     const int32_t *ref_len, const uint8_t *alt_bases, const int64_t *alt_off, const int32_t *alt_len,
+                     const pairhmm_sw_parameters *params, int32_t overhang_strategy, const uint32_t **elems,
+                     const int32_t **n_el, const int32_t **offs) {
     if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
     if (n_pairs < 0 || n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "bad pair count");
     if (!params) return fail(PAIRHMM_ERR_BAD_ARG, "null SW parameters");
@@ -51,8 +51,7 @@ extern "C" int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs,
         if (v > kMaxSwPenalty || v < -kMaxSwPenalty)
             return fail(PAIRHMM_ERR_BAD_ARG, "SW parameters beyond +-1e6");
     if (n_pairs == 0) return PAIRHMM_OK;
-    if (!ref_bases || !ref_off || !ref_len || !alt_bases || !alt_off || !alt_len || !cigars || !cigar_off ||
-        !cigar_cap || !alignment_offset)
+    if (!ref_bases || !ref_off || !ref_len || !alt_bases || !alt_off || !alt_len)
         return fail(PAIRHMM_ERR_BAD_ARG, "null array argument");
     int max_r = 1, max_a = 1;
     int64_t ref_end = 0, alt_end = 0, elem_total = 0;
@@ -138,12 +137,61 @@ extern "C" int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs,
     // stream), so the outputs can come back into the same pinned buffer.
     HIP_TRY(hipMemcpyAsync(hs, ds + out_begin, out_bytes, hipMemcpyDeviceToHost, ctx->stream), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipStreamSynchronize(ctx->stream), PAIRHMM_ERR_KERNEL);
-    const uint32_t *elems = reinterpret_cast<const uint32_t *>(hs);
-    const int32_t *n_el = reinterpret_cast<const int32_t *>(hs + (o_n - out_begin));
-    const int32_t *offs = reinterpret_cast<const int32_t *>(hs + (o_off - out_begin));
+    *elems = reinterpret_cast<const uint32_t *>(hs);
+    *n_el = reinterpret_cast<const int32_t *>(hs + (o_n - out_begin));
+    *offs = reinterpret_cast<const int32_t *>(hs + (o_off - out_begin));
+    return PAIRHMM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases,
+                                          const int64_t *ref_off, const int32_t *ref_len, const uint8_t *alt_bases,
+                                          const int64_t *alt_off, const int32_t *alt_len,
+                                          const pairhmm_sw_parameters *params, int32_t overhang_strategy,
+                                          char *cigars, const int64_t *cigar_off, const int32_t *cigar_cap,
+                                          int32_t *alignment_offset) {
+    if (n_pairs > 0 && (!cigars || !cigar_off || !cigar_cap || !alignment_offset))
+        return fail(PAIRHMM_ERR_BAD_ARG, "null array argument");
+    const uint32_t *elems = nullptr;
+    const int32_t *n_el = nullptr, *offs = nullptr;
+    const int32_t st = sw_run_batch(ctx, n_pairs, ref_bases, ref_off, ref_len, alt_bases, alt_off, alt_len, params,
+                                    overhang_strategy, &elems, &n_el, &offs);
+    if (st != PAIRHMM_OK || n_pairs == 0) return st;
     for (int64_t p = 0, e = 0; p < n_pairs; ++p) {
         if (format_cigar(elems + e, n_el[p], cigars + cigar_off[p], cigar_cap[p]) < 0)
             return fail(PAIRHMM_ERR_BAD_ARG, "CIGAR buffer of pair " + std::to_string(p) + " is too small");
+        alignment_offset[p] = offs[p];
+        e += ref_len[p] + alt_len[p] + 4;
+    }
+    return PAIRHMM_OK;
+}
+
+extern "C" int32_t pairhmm_sw_align_batch_packed(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases,
+                                                 const int64_t *ref_off, const int32_t *ref_len,
+                                                 const uint8_t *alt_bases, const int64_t *alt_off,
+                                                 const int32_t *alt_len, const pairhmm_sw_parameters *params,
+                                                 int32_t overhang_strategy, char *cigar_text, int64_t cigar_text_cap,
+                                                 int64_t *cigar_end, int32_t *alignment_offset) {
+    if (n_pairs > 0 && (!cigar_text || !cigar_end || !alignment_offset))
+        return fail(PAIRHMM_ERR_BAD_ARG, "null array argument");
+    if (cigar_text_cap < 0) return fail(PAIRHMM_ERR_BAD_ARG, "negative CIGAR text capacity");
+    const uint32_t *elems = nullptr;
+    const int32_t *n_el = nullptr, *offs = nullptr;
+    const int32_t st = sw_run_batch(ctx, n_pairs, ref_bases, ref_off, ref_len, alt_bases, alt_off, alt_len, params,
+                                    overhang_strategy, &elems, &n_el, &offs);
+    if (st != PAIRHMM_OK || n_pairs == 0) return st;
+    int64_t pos = 0;
+    for (int64_t p = 0, e = 0; p < n_pairs; ++p) {
+        // format_cigar also writes a terminating 0, which the next CIGAR overwrites; the capacity
+        // check keeps that byte inside the buffer too.
+        const int64_t room = cigar_text_cap - pos;
+        const int32_t w = format_cigar(elems + e, n_el[p], cigar_text + pos,
+                                       static_cast<int32_t>(std::min<int64_t>(room, INT32_MAX)));
+        if (w < 0)
+            return fail(PAIRHMM_ERR_BAD_ARG, "CIGAR text buffer is too small at pair " + std::to_string(p));
+        pos += w;
+        cigar_end[p] = pos;
         alignment_offset[p] = offs[p];
         e += ref_len[p] + alt_len[p] + 4;
     }

@@ -90,19 +90,46 @@ class HipSmithWatermanBinding:
         _raise_for(st, "align")
         return buf.value.decode("ascii"), off.value
 
-    def align_batch(self, refs: Sequence[bytes], alts: Sequence[bytes], parameters: SWParameters,
-                    strategy: SWOverhangStrategy) -> List[Tuple[str, int]]:
+    @staticmethod
+    def _pack(refs, alts):
         n = len(refs)
         if n != len(alts):
             raise ValueError("refs and alts differ in length")
-        if n == 0:
-            return []
         rl = np.array([len(x) for x in refs], np.int32)
         al = np.array([len(x) for x in alts], np.int32)
         ro = np.concatenate([[0], np.cumsum(rl, dtype=np.int64)[:-1]]).astype(np.int64)
         ao = np.concatenate([[0], np.cumsum(al, dtype=np.int64)[:-1]]).astype(np.int64)
         rb = np.frombuffer(b"".join(bytes(x) for x in refs), np.uint8).copy()
         ab = np.frombuffer(b"".join(bytes(x) for x in alts), np.uint8).copy()
+        return n, rl, al, ro, ao, rb, ab
+
+    def align_batch(self, refs: Sequence[bytes], alts: Sequence[bytes], parameters: SWParameters,
+                    strategy: SWOverhangStrategy) -> List[Tuple[str, int]]:
+        """Every pair in one launch (pairhmm_sw_align_batch_packed): the CIGARs come back as one text
+        with end offsets, decoded once."""
+        n, rl, al, ro, ao, rb, ab = self._pack(refs, alts)
+        if n == 0:
+            return []
+        cap = int(_cigar_cap(rl, al).astype(np.int64).sum()) + 1
+        text = np.empty(cap, np.uint8)  # only the used prefix is written (and paged in)
+        ends = np.empty(n, np.int64)
+        offs = np.empty(n, np.int32)
+        p = parameters.native()
+        st = self._lib.pairhmm_sw_align_batch_packed(self._ctx, n, rb.ctypes.data, ro.ctypes.data, rl.ctypes.data,
+                                                     ab.ctypes.data, ao.ctypes.data, al.ctypes.data, ctypes.byref(p),
+                                                     int(strategy), text.ctypes.data, cap, ends.ctypes.data,
+                                                     offs.ctypes.data)
+        _raise_for(st, "align_batch")
+        e = ends.tolist()
+        txt = text[:e[-1]].tobytes().decode("ascii")
+        return [(txt[a:b], f) for a, b, f in zip([0] + e[:-1], e, offs.tolist())]
+
+    def align_batch_buffers(self, refs: Sequence[bytes], alts: Sequence[bytes], parameters: SWParameters,
+                            strategy: SWOverhangStrategy) -> List[Tuple[str, int]]:
+        """The same through pairhmm_sw_align_batch: one NUL-terminated CIGAR buffer per pair."""
+        n, rl, al, ro, ao, rb, ab = self._pack(refs, alts)
+        if n == 0:
+            return []
         caps = _cigar_cap(rl, al).astype(np.int32)
         coff = np.concatenate([[0], np.cumsum(caps, dtype=np.int64)[:-1]]).astype(np.int64)
         cig = np.empty(int(caps.sum()), np.uint8)  # each record is NUL-terminated by the library

@@ -353,6 +353,18 @@ int32_t pairhmm_sw_align_batch(pairhmm_context *ctx, int64_t n_pairs, const uint
                                const pairhmm_sw_parameters *parameters, int32_t overhang_strategy, char *cigars,
                                const int64_t *cigar_off, const int32_t *cigar_cap, int32_t *alignment_offset);
 
+/* Packed batch form (new; no counterpart in the reference): as pairhmm_sw_align_batch, but the CIGARs
+ * are written back to back, without separators, into cigar_text (cigar_text_cap bytes; one byte past
+ * the last CIGAR receives a 0).  Pair p's CIGAR is cigar_text[cigar_end[p-1] .. cigar_end[p]), with
+ * cigar_end[-1] = 0.  A caller decodes the text once instead of one buffer per pair; too small a
+ * buffer is PAIRHMM_ERR_BAD_ARG. */
+int32_t pairhmm_sw_align_batch_packed(pairhmm_context *ctx, int64_t n_pairs, const uint8_t *ref_bases,
+                                      const int64_t *ref_off, const int32_t *ref_len, const uint8_t *alt_bases,
+                                      const int64_t *alt_off, const int32_t *alt_len,
+                                      const pairhmm_sw_parameters *parameters, int32_t overhang_strategy,
+                                      char *cigar_text, int64_t cigar_text_cap, int64_t *cigar_end,
+                                      int32_t *alignment_offset);
+
 #ifdef __cplusplus
 }
 #endif

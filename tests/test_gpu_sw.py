@@ -92,3 +92,37 @@ def test_empty_sequence_is_an_error(binding):
         binding.align(b"", b"ACGT", ORIGINAL_DEFAULT, SWOverhangStrategy.SOFTCLIP)
     with pytest.raises(ValueError):
         binding.align(b"ACGT", b"", ORIGINAL_DEFAULT, SWOverhangStrategy.INDEL)
+
+
+def test_packed_and_per_pair_buffers_agree(binding):
+    """pairhmm_sw_align_batch_packed (one text, end offsets) and pairhmm_sw_align_batch (one buffer
+    per pair) return the same CIGARs and offsets."""
+    rng = np.random.default_rng(11)
+    refs, alts = [], []
+    for k in range(300):
+        H = int(rng.integers(1, 600))
+        hap = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, H)])
+        refs.append(hap)
+        alts.append(_read_from(rng, hap, int(rng.integers(1, 250))))
+    for strategy in SWOverhangStrategy:
+        print(f"[sw] packed vs buffers strategy={strategy.name}", flush=True)
+        assert binding.align_batch(refs, alts, STANDARD_NGS, strategy) == \
+            binding.align_batch_buffers(refs, alts, STANDARD_NGS, strategy)
+
+
+def test_packed_text_too_small_is_an_error(binding):
+    """A CIGAR text buffer that cannot hold every CIGAR (and the closing 0) is rejected after the
+    launch, with no write past its end."""
+    import ctypes
+    refs, alts = [b"ACGTACGTAC", b"AAAACCCCGG"], [b"ACGTTCGTAC", b"CCCC"]
+    n, rl, al, ro, ao, rb, ab = binding._pack(refs, alts)
+    text = np.full(8, 0x7F, np.uint8)  # "10M" fits, the second CIGAR does not
+    ends = np.zeros(n, np.int64)
+    offs = np.zeros(n, np.int32)
+    p = ORIGINAL_DEFAULT.native()
+    st = binding._lib.pairhmm_sw_align_batch_packed(binding._ctx, n, rb.ctypes.data, ro.ctypes.data, rl.ctypes.data,
+                                                    ab.ctypes.data, ao.ctypes.data, al.ctypes.data, ctypes.byref(p),
+                                                    int(SWOverhangStrategy.SOFTCLIP), text.ctypes.data, 5,
+                                                    ends.ctypes.data, offs.ctypes.data)
+    assert st != 0
+    assert (text[5:] == 0x7F).all()
