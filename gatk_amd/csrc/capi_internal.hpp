@@ -40,16 +40,24 @@ inline bool device_is_gfx950(int dev) {
     return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
 }
 
+// Capacity for a buffer of `cap` bytes asked to hold `n` > cap: at least n, and at least 1.5x the
+// old capacity (that step capped at 256 MiB), so a caller whose calls grow a little at a time (one
+// region after another) reallocates a few times rather than at every new maximum.
+inline size_t grown_capacity(size_t cap, size_t n) {
+    const size_t step = std::min<size_t>(cap / 2, size_t(256) << 20);
+    return std::max<size_t>({n, size_t(4096), cap + step});
+}
+
 // Device buffer that only grows.
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
     hipError_t reserve(size_t n) {
         if (n <= cap) return hipSuccess;
+        const size_t want = grown_capacity(cap, n);
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
-        const size_t want = std::max<size_t>(n, 4096);
         hipError_t e = hipMalloc(&ptr, want);
         if (e == hipSuccess) cap = want;
         return e;
@@ -66,10 +74,10 @@ struct HostPinned {
     size_t cap = 0;
     hipError_t reserve(size_t n) {
         if (n <= cap) return hipSuccess;
+        const size_t want = grown_capacity(cap, n);
         if (ptr) (void)hipHostFree(ptr);
         ptr = nullptr;
         cap = 0;
-        const size_t want = std::max<size_t>(n, 4096);
         hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
         if (e == hipSuccess) cap = want;
         return e;
