@@ -13,6 +13,7 @@ of the same pairs on this host's cores and doubles as the accuracy check of that
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -547,6 +548,7 @@ def str_fit_leg(args):
 
 
 def main() -> int:
+    faulthandler.enable()  # a fatal signal prints every thread's Python stack to stderr
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

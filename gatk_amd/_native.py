@@ -176,69 +176,75 @@ def load_library() -> ctypes.CDLL:
             raise NativeLibraryMissing(
                 f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
         L = ctypes.CDLL(LIB_PATH)
-        vp, i32 = ctypes.c_void_p, ctypes.c_int32
-        L.pairhmm_load.argtypes = []
-        L.pairhmm_load.restype = i32
-        L.pairhmm_initialize.argtypes = [ctypes.POINTER(NativeArguments), ctypes.POINTER(vp)]
-        L.pairhmm_initialize.restype = i32
-        L.pairhmm_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
-                                                  ctypes.POINTER(HaplotypeDataHolder), i32, vp]
-        L.pairhmm_compute_likelihoods.restype = i32
-        L.pairhmm_done.argtypes = [vp]
-        L.pairhmm_done.restype = i32
-        L.pairhmm_last_error.argtypes = []
-        L.pairhmm_last_error.restype = ctypes.c_char_p
-        L.pairhmm_batch_upload.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]
-        L.pairhmm_batch_upload.restype = i32
-        L.pairhmm_batch_forward.argtypes = [vp, vp, i32, vp, ctypes.POINTER(RunStats)]
-        L.pairhmm_batch_forward.restype = i32
-        L.pairhmm_batch_free.argtypes = [vp]
-        L.pairhmm_batch_free.restype = i32
-        L.pairhmm_forward_backward.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
-        L.pairhmm_forward_backward.restype = i32
-        L.pairhmm_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
-        L.pairhmm_set_tuning.restype = i32
-        L.pairhmm_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
-        L.pairhmm_get_tuning.restype = i32
-        L.pairhmm_stream_create.argtypes = [vp, ctypes.POINTER(vp)]
-        L.pairhmm_stream_create.restype = i32
-        L.pairhmm_stream_submit.argtypes = [vp, ctypes.POINTER(BatchDesc), vp]
-        L.pairhmm_stream_submit.restype = i32
-        L.pairhmm_stream_finish.argtypes = [vp, ctypes.POINTER(RunStats)]
-        L.pairhmm_stream_finish.restype = i32
-        L.pairhmm_stream_destroy.argtypes = [vp]
-        L.pairhmm_stream_destroy.restype = i32
-        i64 = ctypes.c_int64
-        L.pairhmm_dragstr_analyze.argtypes = [vp, i32, i32, vp, vp]
-        L.pairhmm_dragstr_analyze.restype = i32
-        L.pairhmm_dragstr_contexts.argtypes = [vp, vp, vp, i64, i64, i32, i32, vp]
-        L.pairhmm_dragstr_contexts.restype = i32
-        L.pairhmm_dragstr_impute.argtypes = [vp, vp, i32, i32, vp, vp, vp, i64, i64, vp, vp]
-        L.pairhmm_dragstr_impute.restype = i32
-        L.pairhmm_apply_pcr_error_model.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp]
-        L.pairhmm_apply_pcr_error_model.restype = i32
-        L.pairhmm_cap_read_qualities.argtypes = [vp, vp, vp, i64, vp, i32, i32, vp, vp, vp]
-        L.pairhmm_cap_read_qualities.restype = i32
-        L.pairhmm_dragstr_impute_device.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i64, i32, vp, vp]
-        L.pairhmm_dragstr_impute_device.restype = i32
-        L.pairhmm_pd_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
-                                                     ctypes.POINTER(PDHaplotypeDataHolder), i32, vp]
-        L.pairhmm_pd_compute_likelihoods.restype = i32
-        dbl = ctypes.c_double
-        L.pairhmm_normalize_and_filter_likelihoods.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, vp, vp, dbl,
-                                                               i32, dbl, vp, vp]
-        L.pairhmm_normalize_and_filter_likelihoods.restype = i32
-        L.pairhmm_sw_align.argtypes = [vp, vp, i32, vp, i32, ctypes.POINTER(SWParameters), i32, ctypes.c_char_p,
-                                       i32, ctypes.POINTER(i32)]
-        L.pairhmm_sw_align.restype = i32
-        L.pairhmm_sw_align_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32, vp,
-                                             vp, vp, vp]
-        L.pairhmm_sw_align_batch.restype = i32
-        L.pairhmm_sw_align_batch_packed.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32,
-                                                    vp, i64, vp, vp]
-        L.pairhmm_sw_align_batch_packed.restype = i32
+        declare(L)
         _lib = L
         return L
+
+
+def declare(L: ctypes.CDLL) -> ctypes.CDLL:
+    """Sets the argument and result types of every C-ABI entry point on a loaded library."""
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    L.pairhmm_load.argtypes = []
+    L.pairhmm_load.restype = i32
+    L.pairhmm_initialize.argtypes = [ctypes.POINTER(NativeArguments), ctypes.POINTER(vp)]
+    L.pairhmm_initialize.restype = i32
+    L.pairhmm_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
+                                              ctypes.POINTER(HaplotypeDataHolder), i32, vp]
+    L.pairhmm_compute_likelihoods.restype = i32
+    L.pairhmm_done.argtypes = [vp]
+    L.pairhmm_done.restype = i32
+    L.pairhmm_last_error.argtypes = []
+    L.pairhmm_last_error.restype = ctypes.c_char_p
+    L.pairhmm_batch_upload.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]
+    L.pairhmm_batch_upload.restype = i32
+    L.pairhmm_batch_forward.argtypes = [vp, vp, i32, vp, ctypes.POINTER(RunStats)]
+    L.pairhmm_batch_forward.restype = i32
+    L.pairhmm_batch_free.argtypes = [vp]
+    L.pairhmm_batch_free.restype = i32
+    L.pairhmm_forward_backward.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
+    L.pairhmm_forward_backward.restype = i32
+    L.pairhmm_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pairhmm_set_tuning.restype = i32
+    L.pairhmm_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pairhmm_get_tuning.restype = i32
+    L.pairhmm_stream_create.argtypes = [vp, ctypes.POINTER(vp)]
+    L.pairhmm_stream_create.restype = i32
+    L.pairhmm_stream_submit.argtypes = [vp, ctypes.POINTER(BatchDesc), vp]
+    L.pairhmm_stream_submit.restype = i32
+    L.pairhmm_stream_finish.argtypes = [vp, ctypes.POINTER(RunStats)]
+    L.pairhmm_stream_finish.restype = i32
+    L.pairhmm_stream_destroy.argtypes = [vp]
+    L.pairhmm_stream_destroy.restype = i32
+    i64 = ctypes.c_int64
+    L.pairhmm_dragstr_analyze.argtypes = [vp, i32, i32, vp, vp]
+    L.pairhmm_dragstr_analyze.restype = i32
+    L.pairhmm_dragstr_contexts.argtypes = [vp, vp, vp, i64, i64, i32, i32, vp]
+    L.pairhmm_dragstr_contexts.restype = i32
+    L.pairhmm_dragstr_impute.argtypes = [vp, vp, i32, i32, vp, vp, vp, i64, i64, vp, vp]
+    L.pairhmm_dragstr_impute.restype = i32
+    L.pairhmm_apply_pcr_error_model.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp]
+    L.pairhmm_apply_pcr_error_model.restype = i32
+    L.pairhmm_cap_read_qualities.argtypes = [vp, vp, vp, i64, vp, i32, i32, vp, vp, vp]
+    L.pairhmm_cap_read_qualities.restype = i32
+    L.pairhmm_dragstr_impute_device.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i64, i32, vp, vp]
+    L.pairhmm_dragstr_impute_device.restype = i32
+    L.pairhmm_pd_compute_likelihoods.argtypes = [vp, ctypes.POINTER(ReadDataHolder), i32,
+                                                 ctypes.POINTER(PDHaplotypeDataHolder), i32, vp]
+    L.pairhmm_pd_compute_likelihoods.restype = i32
+    dbl = ctypes.c_double
+    L.pairhmm_normalize_and_filter_likelihoods.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, vp, vp, dbl,
+                                                           i32, dbl, vp, vp]
+    L.pairhmm_normalize_and_filter_likelihoods.restype = i32
+    L.pairhmm_sw_align.argtypes = [vp, vp, i32, vp, i32, ctypes.POINTER(SWParameters), i32, ctypes.c_char_p,
+                                   i32, ctypes.POINTER(i32)]
+    L.pairhmm_sw_align.restype = i32
+    L.pairhmm_sw_align_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32, vp,
+                                         vp, vp, vp]
+    L.pairhmm_sw_align_batch.restype = i32
+    L.pairhmm_sw_align_batch_packed.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(SWParameters), i32,
+                                                vp, i64, vp, vp]
+    L.pairhmm_sw_align_batch_packed.restype = i32
+    return L
 
 
 def last_error() -> str:

@@ -118,16 +118,38 @@ def test_weighted_sum_of_pairs():
     assert _per_read_err(batch, got, want) <= 1e-3
 
 
+def _single_pair_regions(seed, n, hap_len, read_len, sub=0.03):
+    """n regions of one read and one haplotype each: every read belongs to exactly one pair, so a
+    read's gradient is that pair's gradient and compares to the pair's own finite differences."""
+    rng = np.random.default_rng(seed)
+    regions = []
+    for _ in range(n):
+        W = int(rng.integers(*hap_len))
+        hap = rng.choice(list(b"ACGT"), W).astype(np.uint8)
+        R = int(rng.integers(*read_len))
+        st = int(rng.integers(0, max(1, W - R)))
+        b = np.resize(hap[st:st + R], R).copy()
+        flip = rng.random(R) < sub
+        b[flip] = rng.choice(list(b"ACGT"), int(flip.sum())).astype(np.uint8)
+        z = np.zeros(R, np.uint8)
+        regions.append(([hap], [(b, z, z, z, z)]))
+    batch = pack_regions(regions)
+    n_b = len(batch.read_bases)
+    q = {"qb": rng.uniform(10, 40, n_b), "qi": rng.uniform(30, 50, n_b), "qd": rng.uniform(30, 50, n_b),
+         "qc": rng.uniform(8, 15, n_b)}
+    return batch, {k: v.astype(np.float32) for k, v in q.items()}
+
+
 def test_against_finite_differences():
-    """Single-pair problems against FP64 central differences of the forward (1e-3 Phred step)."""
-    batch, q = _problem(11, n_regions=2, hap_len=(40, 120), read_len=(20, 90))
-    out, got, _ = _run(batch, q)
-    for p in range(min(batch.n_pairs, 3)):
+    """Single-pair regions against FP64 central differences of the forward (1e-3 Phred step)."""
+    batch, q = _single_pair_regions(11, 4, hap_len=(40, 120), read_len=(20, 90))
+    assert all(np.sum(batch.pair_read == r) == 1 for r in range(len(batch.read_len)))
+    _, got, _ = _run(batch, q)
+    compared = 0
+    for p in range(batch.n_pairs):
         r, h = batch.pair_read[p], batch.pair_hap[p]
         o, R = int(batch.read_off[r]), int(batch.read_len[r])
         ho, H = int(batch.hap_off[h]), int(batch.hap_len[h])
-        if np.sum(batch.pair_read == r) != 1:
-            continue  # only reads used by exactly one pair compare to a single-pair FD
         hap, read = batch.hap_bases[ho:ho + H], batch.read_bases[o:o + R]
         qs = [q[k][o:o + R].astype(np.float64) for k in NAMES]
         fd = []
@@ -141,7 +163,31 @@ def test_against_finite_differences():
                 col.append((fp - oracle.logless_float(hap, read, *a)) / 2e-3)
             fd.append(np.array(col))
         g = np.concatenate([got[k][o:o + R] for k in NAMES])
-        assert _rel(g, np.concatenate(fd)) <= 1e-3
+        assert _rel(g, np.concatenate(fd)) <= 1e-3, p
+        compared += 1
+    assert compared >= 3
+
+
+def test_batched_uneven_regions_fwd_bwd():
+    """Forward + adjoint on pairs shaped like the 2,000-region differentiation workload: regions of
+    the batched_uneven generator (haplotypes 150-600 bp, reads 100-250 bp), qb = the base
+    qualities, qi = qd = 45, qc = 10.  A few reads per region, each against every kept haplotype,
+    so per-read gradient sums cover several pairs; all against the FP64 oracle adjoint."""
+    full = synth.workload("batched_uneven", n_regions=3, seed=3)
+    regions = []
+    for k in range(full.n_regions):
+        reads, haps = full.region(k)
+        regions.append((haps[:6], reads[:3]))
+    batch = pack_regions(regions)
+    assert batch.n_pairs >= 20 and int(batch.hap_len.max()) >= 150
+    n = len(batch.read_bases)
+    q = {"qb": batch.read_quals.astype(np.float32), "qi": np.full(n, 45.0, np.float32),
+         "qd": np.full(n, 45.0, np.float32), "qc": np.full(n, 10.0, np.float32)}
+    want_out, want = _oracle(batch, q)
+    out, got, st = _run(batch, q)
+    assert np.max(np.abs(out - want_out)) <= 1e-4
+    assert _per_read_err(batch, got, want) <= 1e-3
+    assert _rel(got["eb"], want["eb"]) <= 1e-3
 
 
 def test_precision_stress_pairs_go_to_fp64():

@@ -106,6 +106,46 @@ def test_plain_small_drop_in(hmmresults):
         assert np.max(np.abs(out.reshape(len(reads), len(haps)) - want)) <= tol
 
 
+def test_matrix_allele_order_is_a_permutation_of_the_initialize_order():
+    """VectorLoglessPairHMM.computeLog10Likelihoods (VectorLoglessPairHMM.java:140-153): the engine
+    writes read-major results in the order of the haplotypes given to initialize, and each matrix
+    allele takes the value at its own haplotype's index there, whatever the matrix's allele order."""
+    b = synth.workload("plain_small")
+    reads, haps = b.region(0)
+    reads, haps = reads[:12], haps[:7]
+    want = np.array([[oracle.logless(h, *r) for h in haps] for r in reads])  # [read, init order]
+    perm = np.random.default_rng(0).permutation(len(haps))
+    assert (perm != np.arange(len(haps))).all()
+    hmm = HipLoglessPairHMM(PairHMMNativeArguments(1, False))
+    try:
+        hmm.initialize([bytes(h) for h in haps], None, 0, 0)
+        matrix = LikelihoodMatrix([bytes(haps[i]) for i in perm], len(reads))
+        hmm.compute_log10_likelihoods(matrix, [Read(r[0], r[1], r[2], r[3]) for r in reads], _InOrder(reads))
+        la = hmm.get_log_likelihood_array().reshape(len(reads), len(haps))
+    finally:
+        hmm.close()
+    assert np.max(np.abs(la - want)) <= FP32_TOL  # engine output: initialize order
+    for a, i in enumerate(perm):
+        for r in range(len(reads)):
+            assert matrix.get(a, r) == la[r, i]
+    assert np.max(np.abs(matrix.values - want[:, perm].T)) <= FP32_TOL
+
+
+class _InOrder:
+    """Hands out each read's own insertion / deletion / GCP planes in call order (impute is called
+    once per read, in list order: VectorLoglessPairHMM.java:120-129)."""
+
+    def __init__(self, reads):
+        self.reads, self.k = reads, 0
+
+    def impute(self, read):
+        from gatk_amd.pairhmm import PairHMMInputScoreImputation
+        r = self.reads[self.k]
+        self.k += 1
+        assert bytes(r[0]) == bytes(read.bases)
+        return PairHMMInputScoreImputation(r[2], r[3], r[4])
+
+
 def _edge_region(rng):
     hap = rng.choice(list(b"ACGT"), 300).astype(np.uint8)
     haps = [hap, hap[:1], hap[:20], hap[:64], hap[:65], np.frombuffer(b"ACGTNNACGTacgt" * 5, np.uint8)]

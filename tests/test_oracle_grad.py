@@ -83,3 +83,46 @@ def test_exact_match_closed_form():
     e = 10 ** (-2.5)
     want = -1.0 / ((1 - e) * math.log(10))
     assert np.max(np.abs(g["eb"] / want - 1.0)) <= 1e-6
+
+
+def test_continuous_forward_equals_logless_at_integer_phred():
+    """At integer Phred the continuous model (MM = 1 - e_ins - e_del) is LoglessPairHMM itself: the
+    Jacobian log-sum table behind matchToMatchProb is exact at integer Phred differences
+    (MathUtils.java:406-423, PairHMMModel.java:373-388).  200 random pairs with |q_ins - q_del| < 80
+    agree to 1e-10 with the pinned FP64 Logless restatement."""
+    rng = np.random.default_rng(77)
+    worst = 0.0
+    for _ in range(200):
+        H, R = int(rng.integers(10, 120)), int(rng.integers(3, 90))
+        hap, read, _ = _pair(rng, H, R)
+        qb = rng.integers(6, 41, R)
+        qi = rng.integers(20, 60, R)
+        qd = rng.integers(20, 60, R)
+        qc = rng.integers(5, 20, R)
+        want = oracle.logless(hap, read, qb, qi, qd, qc)
+        got = oracle.logless_float(hap, read, *(x.astype(np.float64) for x in (qb, qi, qd, qc)))
+        worst = max(worst, abs(got - want))
+    assert worst <= 1e-10, worst
+
+
+def test_continuous_forward_at_the_table_drop_rule():
+    """|q_ins - q_del| >= 80: matchToMatchProb drops the smaller error term (MathUtils.java:409,479),
+    so Java's MM exceeds the continuous 1 - e_ins - e_del by e_small on those rows.  Everything else
+    coincides, so 0 < log10 L_java - log10 L_float <= sum over rows of e_small / (MM ln 10)."""
+    rng = np.random.default_rng(78)
+    for _ in range(20):
+        H, R = int(rng.integers(20, 80)), int(rng.integers(5, 40))
+        hap, read, _ = _pair(rng, H, R)
+        qb = rng.integers(10, 41, R)
+        qi = rng.integers(5, 15, R)
+        qd = qi + rng.integers(80, 100, R)
+        qc = rng.integers(5, 20, R)
+        java = oracle.logless(hap, read, qb, qi, qd, qc)
+        cont = oracle.logless_float(hap, read, *(x.astype(np.float64) for x in (qb, qi, qd, qc)))
+        e_small = 10.0 ** (-qd / 10.0)
+        mm = 1.0 - 10.0 ** (-qi / 10.0) - e_small
+        bound = float(np.sum(e_small / mm)) / math.log(10)
+        assert 0.0 < java - cont <= bound * (1 + 1e-6) + 1e-13, (java - cont, bound)
+        # one step inside the rule (|q_ins - q_del| = 79) nothing is dropped and the two agree
+        assert abs(oracle.logless(hap, read, qb, qi, qi + 79, qc)
+                   - oracle.logless_float(hap, read, *(x.astype(np.float64) for x in (qb, qi, qi + 79, qc)))) <= 1e-10
