@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "pairhmm_batch_forward",
     "pairhmm_batch_free",
     "pairhmm_forward_backward",
+    "pairhmm_forward_backward_noalloc",
     "pairhmm_set_tuning",
     "pairhmm_get_tuning",
     "pairhmm_stream_create",
@@ -203,6 +204,8 @@ def declare(L: ctypes.CDLL) -> ctypes.CDLL:
     L.pairhmm_batch_free.restype = i32
     L.pairhmm_forward_backward.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
     L.pairhmm_forward_backward.restype = i32
+    L.pairhmm_forward_backward_noalloc.argtypes = [vp, ctypes.POINTER(GradDesc), ctypes.POINTER(RunStats)]
+    L.pairhmm_forward_backward_noalloc.restype = i32
     L.pairhmm_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.pairhmm_set_tuning.restype = i32
     L.pairhmm_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]

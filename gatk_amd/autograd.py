@@ -62,7 +62,7 @@ class GradProblem:
 class GradEngine:
     """One engine context for forward + backward calls."""
 
-    def __init__(self, use_double: bool = False) -> None:
+    def __init__(self, use_double: bool = False, no_alloc: bool = False) -> None:
         self.lib = _native.load_library()
         if not self.lib.pairhmm_load():
             raise HardwareFeatureException("no gfx950 device visible to this process")
@@ -70,6 +70,8 @@ class GradEngine:
         self.ctx = ctypes.c_void_p()
         _raise_for(self.lib.pairhmm_initialize(ctypes.byref(na), ctypes.byref(self.ctx)), "initialize")
         self.use_double = use_double
+        # no_alloc: pairhmm_forward_backward_noalloc (fixed device scratch, no allocation per call)
+        self.no_alloc = no_alloc
         self.last_stats = None
 
     def run(self, p: GradProblem, qb, qi, qd, qc, weight: Optional[torch.Tensor] = None, want_eb: bool = False):
@@ -103,7 +105,8 @@ class GradEngine:
         d.grad_base_error = grads["eb"].data_ptr() if want_eb else None
         st = _native.RunStats()
         torch.cuda.synchronize(dev)  # inputs were written on torch's stream; the engine uses its own
-        _raise_for(self.lib.pairhmm_forward_backward(self.ctx, ctypes.byref(d), ctypes.byref(st)), "forward_backward")
+        fn = self.lib.pairhmm_forward_backward_noalloc if self.no_alloc else self.lib.pairhmm_forward_backward
+        _raise_for(fn(self.ctx, ctypes.byref(d), ctypes.byref(st)), "forward_backward")
         self.last_stats = st
         return out, grads
 

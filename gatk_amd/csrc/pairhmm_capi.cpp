@@ -677,7 +677,20 @@ extern "C" int32_t pairhmm_batch_free(pairhmm_batch *bt) {
 // Forward + backward
 // ---------------------------------------------------------------------------------------------
 
-extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *d, pairhmm_run_stats *stats) {
+namespace {
+
+// Fixed device memory a forward + backward pass carves its work lists and checkpoints from instead of
+// the context's growable buffers (pairhmm_forward_backward_noalloc).
+struct FixedScratch {
+    uint8_t *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// pairhmm_forward_backward over the context's buffers (fixed == nullptr), or over `fixed`: then
+// nothing is allocated, no event is created, and each launch's persistent grid is held to the
+// waves whose checkpoints fit.
+int32_t forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *d, pairhmm_run_stats *stats,
+                         const FixedScratch *fixed) {
     if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
     if (!d) return fail(PAIRHMM_ERR_BAD_ARG, "null descriptor");
     if (d->n_pairs < 0 || d->n_reads < 0 || d->n_haps < 0 || d->read_bytes < 0)
@@ -704,21 +717,32 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     const uint32_t np = static_cast<uint32_t>(d->n_pairs);
     // Work area: 64 counter slots ([1] FP64 item count, [2] FP64 head, [16+k] pairs of class k,
     // [32+r] scatter cursor of run r, [48+r] queue head of run r), the FP64 list, the class order.
-    if (ctx->grad_work.reserve(sizeof(uint32_t) * (64 + 2 * static_cast<size_t>(np))) != hipSuccess)
-        return fail(PAIRHMM_ERR_OOM, "adjoint work-list allocation failed");
-    uint32_t *counters = static_cast<uint32_t *>(ctx->grad_work.ptr);
+    const size_t work_bytes = sizeof(uint32_t) * (64 + 2 * static_cast<size_t>(np));
+    uint32_t *counters = nullptr;
+    if (fixed) {
+        if (align_up(work_bytes, 256) >= fixed->bytes)
+            return fail(PAIRHMM_ERR_BAD_ARG, std::to_string(np) + " pairs exceed the fixed scratch (" +
+                                                 std::to_string(fixed->bytes) + " bytes)");
+        counters = reinterpret_cast<uint32_t *>(fixed->ptr);
+    } else {
+        if (ctx->grad_work.reserve(work_bytes) != hipSuccess)
+            return fail(PAIRHMM_ERR_OOM, "adjoint work-list allocation failed");
+        counters = static_cast<uint32_t *>(ctx->grad_work.ptr);
+    }
     uint32_t *rescue = counters + 64;
     uint32_t *order = rescue + np;
     hipStream_t s = ctx->stream;
     hipEvent_t ev[3] = {};
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
+    if (!fixed)
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e), PAIRHMM_ERR_KERNEL);
     struct EvGuard {
         hipEvent_t *e;
         ~EvGuard() {
-            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+            for (int i = 0; i < 3; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
         }
     } guard{ev};
-    HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
+    if (!fixed) HIP_TRY(hipEventRecord(ev[0], s), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_zero_counters(counters, 64, s), PAIRHMM_ERR_KERNEL);
 
     // FP32 launches: one per run of LDS classes, largest class first; classes with few pairs merge
@@ -730,7 +754,18 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
         LaunchShape shape;
         int grid;
     };
-    std::vector<Run> runs;
+    struct RunList {
+        Run item[kNumColClasses];
+        size_t n = 0;
+        size_t size() const { return n; }
+        void push_back(const Run &r) { item[n++] = r; }
+        Run &operator[](size_t i) { return item[i]; }
+        const Run &operator[](size_t i) const { return item[i]; }
+        Run *begin() { return item; }
+        Run *end() { return item + n; }
+        const Run *begin() const { return item; }
+        const Run *end() const { return item + n; }
+    } runs;  // at most one run per LDS class; fixed storage
     ClassRuns cr{};
     if (!d->use_double) {
         HIP_TRY(launch_grad_class_count(d->pair_hap, d->hap_len, np, max_class, counters + 16, s), PAIRHMM_ERR_KERNEL);
@@ -739,19 +774,20 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
         HIP_TRY(hipStreamSynchronize(s), PAIRHMM_ERR_KERNEL);
         const uint32_t merge_below = std::max<uint32_t>(4096, np / 64);
         uint32_t pending = 0;
-        std::vector<int> members;
+        int members[kNumColClasses];
+        int n_members = 0;
         for (int k = 0; k < kNumColClasses; ++k) cr.run_of_class[k] = -1;
         for (int k = 0; k <= max_class; ++k) {
             if (counts[k] == 0) continue;
             pending += counts[k];
-            members.push_back(k);
+            members[n_members++] = k;
             bool later = false;
             for (int j = k + 1; j <= max_class; ++j) later |= counts[j] > 0;
             if (pending >= merge_below || !later) {
-                for (int m : members) cr.run_of_class[m] = static_cast<int32_t>(runs.size());
+                for (int i = 0; i < n_members; ++i) cr.run_of_class[members[i]] = static_cast<int32_t>(runs.size());
                 runs.push_back(Run{k, 0, pending, LaunchShape{}, 0});
                 pending = 0;
-                members.clear();
+                n_members = 0;
             }
         }
         uint32_t acc = 0;  // largest class first in the order list
@@ -770,13 +806,37 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     int max_stripes = 0, max_tiles = 0;
     const size_t per_wave = grad_scratch_elems_per_wave(d->max_read_len, d->max_hap_len, &max_stripes, &max_tiles);
     const int cus = std::max(ctx->num_cus, 1);
-    const int grid64 = sh64.blocks_per_cu * cus;
-    size_t waves32 = 0;
-    for (const Run &r : runs) waves32 = std::max(waves32, static_cast<size_t>(r.grid) * r.shape.waves_per_block);
-    const size_t waves64 = static_cast<size_t>(grid64) * sh64.waves_per_block;
-    if (ctx->grad_scratch32.reserve(std::max<size_t>(waves32, 1) * per_wave * sizeof(float)) != hipSuccess ||
-        ctx->grad_scratch64.reserve(waves64 * per_wave * sizeof(double)) != hipSuccess)
-        return fail(PAIRHMM_ERR_OOM, "adjoint scratch allocation failed");
+    int grid64 = sh64.blocks_per_cu * cus;
+    void *scratch32 = nullptr, *scratch64 = nullptr;
+    if (fixed) {
+        // The FP32 runs and the FP64 launch follow each other on one stream, so their checkpoints
+        // share the space after the work area; each grid is held to the blocks whose waves fit.
+        const size_t ck_off = align_up(work_bytes, 256);
+        const size_t ck_bytes = fixed->bytes - ck_off;
+        auto blocks_fit = [&](const LaunchShape &sh, size_t elem) {
+            return ck_bytes / (static_cast<size_t>(sh.waves_per_block) * per_wave * elem);
+        };
+        if (blocks_fit(sh64, sizeof(double)) < 1)
+            return fail(PAIRHMM_ERR_BAD_ARG, "reads of " + std::to_string(d->max_read_len) + " and haplotypes of " +
+                                                 std::to_string(d->max_hap_len) +
+                                                 " bases need more checkpoint space than the fixed scratch holds");
+        grid64 = static_cast<int>(std::min<size_t>(static_cast<size_t>(grid64), blocks_fit(sh64, sizeof(double))));
+        for (Run &r : runs) {
+            const size_t fit = blocks_fit(r.shape, sizeof(float));
+            if (fit < 1) return fail(PAIRHMM_ERR_BAD_ARG, "adjoint checkpoints exceed the fixed scratch");
+            r.grid = static_cast<int>(std::min<size_t>(static_cast<size_t>(r.grid), fit));
+        }
+        scratch32 = scratch64 = fixed->ptr + ck_off;
+    } else {
+        size_t waves32 = 0;
+        for (const Run &r : runs) waves32 = std::max(waves32, static_cast<size_t>(r.grid) * r.shape.waves_per_block);
+        const size_t waves64 = static_cast<size_t>(grid64) * sh64.waves_per_block;
+        if (ctx->grad_scratch32.reserve(std::max<size_t>(waves32, 1) * per_wave * sizeof(float)) != hipSuccess ||
+            ctx->grad_scratch64.reserve(waves64 * per_wave * sizeof(double)) != hipSuccess)
+            return fail(PAIRHMM_ERR_OOM, "adjoint scratch allocation failed");
+        scratch32 = ctx->grad_scratch32.ptr;
+        scratch64 = ctx->grad_scratch64.ptr;
+    }
     GradBatch g{};
     g.hap_bases = d->hap_bases;
     g.hap_off = d->hap_off;
@@ -808,12 +868,12 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
     if (d->grad_base_error) HIP_TRY(hipMemsetAsync(d->grad_base_error, 0, gbytes, s), PAIRHMM_ERR_KERNEL);
     RescueSink none{nullptr, nullptr, 0.0};
     if (d->use_double) {
-        HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
-        g.scratch = ctx->grad_scratch64.ptr;
+        if (!fixed) HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
+        g.scratch = scratch64;
         WorkSource all{nullptr, static_cast<uint32_t>(d->n_pairs), nullptr, counters + 2};
         HIP_TRY(launch_grad(true, g, all, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
     } else {
-        g.scratch = ctx->grad_scratch32.ptr;
+        g.scratch = scratch32;
         RescueSink rs{rescue, counters + 1, kFp32RescueThreshold};  // as the forward, plus the adjoint's range
         for (size_t r = runs.size(); r-- > 0;) {
             const Run &q = runs[r];
@@ -822,22 +882,74 @@ extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_
                                 q.shape, q.grid, s),
                     PAIRHMM_ERR_KERNEL);
         }
-        HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
-        g.scratch = ctx->grad_scratch64.ptr;
+        if (!fixed) HIP_TRY(hipEventRecord(ev[1], s), PAIRHMM_ERR_KERNEL);
+        g.scratch = scratch64;
         WorkSource again{rescue, 0, counters + 1, counters + 2};
         HIP_TRY(launch_grad(true, g, again, none, d->max_hap_len, sh64, grid64, s), PAIRHMM_ERR_KERNEL);
     }
-    HIP_TRY(hipEventRecord(ev[2], s), PAIRHMM_ERR_KERNEL);
+    if (!fixed) HIP_TRY(hipEventRecord(ev[2], s), PAIRHMM_ERR_KERNEL);
     uint32_t host_counters[4] = {};
     HIP_TRY(hipMemcpyAsync(host_counters, counters, sizeof(host_counters), hipMemcpyDeviceToHost, s), PAIRHMM_ERR_KERNEL);
     HIP_TRY(hipStreamSynchronize(s), PAIRHMM_ERR_KERNEL);
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev[0], ev[2]) == hipSuccess) st.total_ms = ms;
-    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.fp32_kernel_ms = ms;
-    if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
+    if (!fixed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[0], ev[2]) == hipSuccess) st.total_ms = ms;
+        if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.fp32_kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
+    }
     st.n_rescued = d->use_double ? d->n_pairs : host_counters[1];
     if (stats) *stats = st;
     return PAIRHMM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *d, pairhmm_run_stats *stats) {
+    return forward_backward(ctx, d, stats, nullptr);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward + backward without allocation
+// ---------------------------------------------------------------------------------------------
+
+// The fixed scratch of pairhmm_forward_backward_noalloc: one array per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, once per device, and only that address is used.
+static constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on the first call there).
+int32_t device_scratch(int device, uint8_t **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(PAIRHMM_ERR_STATE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(PAIRHMM_ERR_KERNEL, std::string("scratch symbol lookup: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return PAIRHMM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t pairhmm_forward_backward_noalloc(pairhmm_context *ctx, const pairhmm_grad_desc *d,
+                                                    pairhmm_run_stats *stats) {
+    if (!ctx) return fail(PAIRHMM_ERR_STATE, "context is null");
+    HIP_TRY(hipSetDevice(ctx->device), PAIRHMM_ERR_KERNEL);
+    FixedScratch fx;
+    const int32_t st = device_scratch(ctx->device, &fx.ptr);
+    if (st != PAIRHMM_OK) return st;
+    fx.bytes = kNoAllocScratchBytes;
+    return forward_backward(ctx, d, stats, &fx);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -203,6 +203,21 @@ typedef struct pairhmm_grad_desc {
  * the FP32 adjoint range are recomputed in FP64.  Synchronous. */
 int32_t pairhmm_forward_backward(pairhmm_context *ctx, const pairhmm_grad_desc *desc, pairhmm_run_stats *stats);
 
+/* pairhmm_forward_backward without allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and outputs as pairhmm_forward_backward, but the
+ * work lists and checkpoints live in one fixed device array of 256 MiB per GPU (allocated when the
+ * library's code object is loaded on that GPU and held until the process ends) instead of the
+ * context's growable buffers; no HIP allocation, free or event creation happens in the call.  Each
+ * persistent grid is held to the waves whose checkpoints fit, so large problems run at lower
+ * occupancy.  Fails with PAIRHMM_ERR_BAD_ARG when the pair list (4 bytes x (64 + 2 n_pairs)) or
+ * one workgroup's checkpoints (set by max_read_len and max_hap_len) exceed the array.  Every call on
+ * a GPU shares that GPU's array and leaves its contents behind: calls that overlap on one GPU (two
+ * threads, two contexts) race and must be serialised by the caller.  stats->total_ms and the kernel
+ * times are 0 (no events are created). */
+int32_t pairhmm_forward_backward_noalloc(pairhmm_context *ctx, const pairhmm_grad_desc *desc,
+                                         pairhmm_run_stats *stats);
+
 /* ---- DRAGstr input side (host code; no device needed) -------------------------------------- */
 
 /* Per read position: the period (1..max_period) with the most tandem copies overlapping the

@@ -28,6 +28,10 @@ typedef ihipGraph *hipGraph_t;
 typedef hipGraphExec *hipGraphExec_t;
 typedef struct ihipGraphNode *hipGraphNode_t;
 
+#define __device__
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
+
 #define hipStreamNonBlocking 0x01
 #define hipEventDisableTiming 0x02
 #define hipHostMallocDefault 0x0
@@ -103,3 +107,7 @@ inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, hipGraphNode
 inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+    *p = const_cast<void *>(symbol);
+    return hipSuccess;
+}
