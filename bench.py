@@ -564,6 +564,8 @@ def main() -> int:
                     help="regions of the STR penalty fit leg (100 reads each; 0 skips it)")
     ap.add_argument("--fit-iters", type=int, default=200)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-schedule autotuner")
+    ap.add_argument("--stripes", choices=("packed", "aligned"), default=None,
+                    help="force the FP32 stripe schedule of the headline leg (with --no-tune)")
     ap.add_argument("--stream-regions", type=int, default=0,
                     help="regions of the double-buffered stream-API leg with a generator process pool "
                          "(off by default; see the streaming notes of the design document)")
@@ -630,6 +632,11 @@ def main() -> int:
         schedule = autotune.tune(batch, use_cache=False, cache_path=None)
         schedule.apply(eng)
         log(f"autotuner picked {schedule} in {time.perf_counter() - t0:.1f} s")
+    elif args.stripes:
+        from gatk_amd import autotune
+
+        schedule = autotune.Schedule(stripe_schedule=0 if args.stripes == "packed" else 1)
+        schedule.apply(eng)
     t0 = time.perf_counter()
     dev = eng.upload(batch)
     log(f"uploaded to HBM in {time.perf_counter() - t0:.1f} s")

@@ -1,9 +1,11 @@
 """Autotuner of the batched forward launch schedule.
 
-The persistent forward kernel has three schedule knobs (``pairhmm_tuning`` in
+The persistent forward kernel has four schedule knobs (``pairhmm_tuning`` in
 include/gatk_pairhmm.h): wavefronts per workgroup (which sets how the per-wave LDS slices pack into
-a CU's 160 KB), the persistent grid as a fraction of resident capacity, and the work-queue order
-(descending cost vs submission order).  ``tune`` uploads a sample of the workload, times every
+a CU's 160 KB), the persistent grid as a fraction of resident capacity, the work-queue order
+(descending cost vs submission order), and the stripe schedule of the FP32 kernel: packed stripes
+(the rows of consecutive pairs share a wave's 64-row stripes) or aligned stripes (one pair per
+stripe, the read's last stripe padded to 64 rows).  ``tune`` uploads a sample of the workload, times every
 candidate's FP32 kernel with HIP events (eager launches, best of ``repeats``), and returns the
 fastest.  Results are cached in a JSON file keyed by device and a coarse workload signature
 (haplotype / read length classes), so later runs on the same device and workload shape reuse them.
@@ -31,14 +33,15 @@ class Schedule:
     waves_per_block: int = 4
     grid_percent: int = 100
     queue_order: int = 0
+    stripe_schedule: int = 0  # 0 packed stripes, 1 aligned stripes
 
     def apply(self, engine: BatchEngine) -> None:
-        t = _native.Tuning(self.waves_per_block, self.grid_percent, self.queue_order)
+        t = _native.Tuning(self.waves_per_block, self.grid_percent, self.queue_order, self.stripe_schedule)
         _raise_for(engine.lib.pairhmm_set_tuning(engine.ctx, ctypes.byref(t)), "set_tuning")
 
 
-CANDIDATES: List[Schedule] = [Schedule(w, g, o) for w, g, o in
-                              itertools.product((1, 2, 4, 8), (50, 100, 200), (0, 1))]
+CANDIDATES: List[Schedule] = [Schedule(w, g, o, s) for w, g, o, s in
+                              itertools.product((1, 2, 4, 8), (50, 100, 200), (0, 1), (0, 1))]
 
 
 def workload_signature(batch: PackedBatch) -> str:

@@ -10,7 +10,9 @@ reshaped to (reads, haplotypes) is region k's likelihood matrix.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import weakref
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -158,6 +160,20 @@ class RunStats:
         return self.cells / (self.total_ms * 1e-3) / 1e9 if self.total_ms > 0 else float("nan")
 
 
+# Engines still open when the interpreter exits (a failed test keeps its locals alive in the
+# traceback) are closed by an atexit hook, while the HIP runtime is still up.
+_LIVE_ENGINES: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_engines() -> None:
+    for eng in list(_LIVE_ENGINES):
+        try:
+            eng.close()
+        except Exception:
+            pass
+
+
 class DeviceBatch:
     """A PackedBatch resident in HBM (pairhmm_batch_upload) with its work queue."""
 
@@ -168,6 +184,7 @@ class DeviceBatch:
         self._h = ctypes.c_void_p()
         d = batch.desc()
         _raise_for(engine.lib.pairhmm_batch_upload(engine.ctx, ctypes.byref(d), ctypes.byref(self._h)), "batch_upload")
+        engine._batches.add(self)  # a batch is freed before its engine's context (pairhmm_batch_free reads it)
 
     def forward(self, mode: int = _native.RUN_PERSISTENT, out: Optional[np.ndarray] = None) -> RunStats:
         st = _native.RunStats()
@@ -203,12 +220,19 @@ class BatchEngine:
         na = _native.NativeArguments(int(a.max_number_of_threads), int(bool(a.use_double_precision)))
         self.ctx = ctypes.c_void_p()
         _raise_for(self.lib.pairhmm_initialize(ctypes.byref(na), ctypes.byref(self.ctx)), "initialize")
+        self._batches: "weakref.WeakSet" = weakref.WeakSet()  # released before the context
+        self._streams: "weakref.WeakSet" = weakref.WeakSet()
+        _LIVE_ENGINES.add(self)
 
     def upload(self, batch: PackedBatch) -> DeviceBatch:
         return DeviceBatch(self, batch)
 
     def close(self) -> None:
         if self.ctx:
+            for st in list(self._streams):
+                st.close()
+            for bt in list(self._batches):
+                bt.free()
             self.lib.pairhmm_done(self.ctx)
             self.ctx = ctypes.c_void_p()
 

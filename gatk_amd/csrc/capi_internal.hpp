@@ -91,6 +91,13 @@ struct HostPinned {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// The packed-stripe FP32 kernel reserves the haplotype bytes 0 (padding) and 1 (the row-0 marker);
+// haplotype planes holding either run the aligned schedule.
+inline bool haps_packable(const uint8_t *hap, int64_t n) {
+    return n <= 0 || (std::memchr(hap, 0, static_cast<size_t>(n)) == nullptr &&
+                      std::memchr(hap, 1, static_cast<size_t>(n)) == nullptr);
+}
+
 // PairHMM.computeReadLikelihoodGivenHaplotypeLog10 argument checks (PairHMM.java:284-292) and
 // PairHMMModel.qualToTransProbs sign checks (PairHMMModel.java:108-111).
 inline int32_t validate_read(const uint8_t *b, const uint8_t *q, const uint8_t *ins, const uint8_t *del, const uint8_t *gcp,
@@ -144,9 +151,15 @@ constexpr double kDirectFp64Score = -90.0;
 
 // Validates a host batch descriptor and builds its work queue and slices.  Fills cells (sum of
 // R*H) and the longest haplotype.  With route_direct, pairs whose read scores below
-// kDirectFp64Score are placed at the end of their slice (QueueSlice::n_direct).
+// kDirectFp64Score are placed at the end of their slice (QueueSlice::n_direct).  Queue order 0
+// sorts each slice by descending cost for the aligned schedule, and by descending haplotype length
+// for the packed one (by_hap_len), whose stripes last as long as their pairs' longest haplotype.
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_hap_len, std::vector<QueueSlice> &slices, bool route_direct = false);
+                           int &max_hap_len, std::vector<QueueSlice> &slices, bool route_direct = false,
+                           bool by_hap_len = false);
+
+// The packed schedule for this context and haplotype plane.
+inline bool use_packed(const pairhmm_context *ctx, bool packable);
 
 // True when a read's expected log10 emission along a perfect alignment,
 // sum_i (1-e_i) log10(1-e_i) + e_i log10(e_i/3) over its base-quality error rates (each term rounded
@@ -178,6 +191,7 @@ struct pairhmm_context {
     int waves_per_block = 4;  // launch schedule (pairhmm_set_tuning)
     int grid_percent = 100;
     int queue_order = 0;
+    int stripe_schedule = 0;  // 0 packed stripes, 1 aligned stripes (FP32 pass)
 };
 
 namespace gatk_pairhmm {
@@ -192,7 +206,9 @@ int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_
 // offset and capacity in the queue).
 hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
-                              uint32_t *rescue_list, hipStream_t s);
+                              uint32_t *rescue_list, hipStream_t s, bool packed);
+
+inline bool use_packed(const pairhmm_context *ctx, bool packable) { return packable && ctx->stripe_schedule == 0; }
 
 // Enqueues the FP64 forward kernel once per slice, each with its own slice's LDS class: over the
 // slice's rescue list (rescue_pass, the count read on the device), or over all of its queue.  The

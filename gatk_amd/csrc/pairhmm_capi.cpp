@@ -33,6 +33,7 @@ struct pairhmm_batch {
     std::vector<QueueSlice> slices; // FP32 and FP64 launches over the queue, one per LDS class run
     double *d_f64 = nullptr;        // FP64 boundary planes of the batch's FP64 launches
     size_t f64_bytes = 0;
+    bool packed = false;            // FP32 pass on packed stripes (fixed at upload with the queue order)
 };
 
 extern "C" {
@@ -210,7 +211,9 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     uint32_t *counters = reinterpret_cast<uint32_t *>(ds + o_cnt);
     uint32_t *rescue = reinterpret_cast<uint32_t *>(ds + o_rescue);
 
-    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block);
+    bool packable = true;
+    for (int32_t h = 0; h < n_haps && packable; ++h) packable = haps_packable(haps[h].haplotype_bases, haps[h].length);
+    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, use_packed(ctx, packable));
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
     if (sh64.waves_per_block == 0 || sh32.waves_per_block == 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
@@ -220,9 +223,10 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
         return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, static_cast<int64_t>(sh.blocks_per_cu) * cus)));
     };
     WorkSource all{nullptr, static_cast<uint32_t>(n_pairs), nullptr, nullptr};
-    const WorkSource fp32_items = n_direct ? WorkSource{reinterpret_cast<const uint32_t *>(ds + o_order),
-                                                        static_cast<uint32_t>(n32), nullptr, nullptr}
-                                           : all;
+    // The FP32 pass takes its pairs from a queue whose head is counters[0] (zero in the staged
+    // counters); the packed schedule takes them in chunks from it.
+    const WorkSource fp32_items{n_direct ? reinterpret_cast<const uint32_t *>(ds + o_order) : nullptr,
+                                static_cast<uint32_t>(n32), nullptr, counters};
     RescueSink none{nullptr, nullptr, 0.0};
     // FP64 boundary planes for every wave of the FP64 grid (a grid-stride launch needs them all).
     const int grid64 = grid_for(sh64);
@@ -286,7 +290,7 @@ void direct_rescue_init(const std::vector<QueueSlice> &slices, const std::vector
 }
 
 int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::vector<uint32_t> &order, int64_t &cells,
-                           int &max_h, std::vector<QueueSlice> &slices, bool route_direct) {
+                           int &max_h, std::vector<QueueSlice> &slices, bool route_direct, bool by_hap_len) {
     if (d->n_reads < 0 || d->n_haps <= 0 || d->n_pairs < 0 || d->read_bytes < 0 || d->hap_bytes <= 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "bad batch sizes");
     if (d->n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "batch exceeds 2^32 pairs");
@@ -321,8 +325,10 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
             return fail(PAIRHMM_ERR_BAD_ARG, "pair " + std::to_string(p) + " indexes outside the batch");
         const int R = d->read_len[r], H = d->hap_len[h];
         cells += static_cast<int64_t>(R) * H;
-        // wave-steps of the pair: stripes of 63 rows, each H + 64 steps long
-        const uint32_t c = static_cast<uint32_t>((R + kWave - 2) / (kWave - 1)) * static_cast<uint32_t>(H + kWave);
+        // Aligned schedule: the pair's wave-steps, ceil(R/64) stripes of H + 64 steps.  Packed: the
+        // haplotype length, so the pairs a wave takes one after another share stripe lengths.
+        const uint32_t c = by_hap_len ? static_cast<uint32_t>(H)
+                                      : static_cast<uint32_t>((R + kWave - 1) / kWave) * static_cast<uint32_t>(H + kWave);
         cost[p] = c;
         max_cost = std::max(max_cost, c);
     }
@@ -425,11 +431,11 @@ int persistent_grid(const pairhmm_context *ctx, const LaunchShape &shape, int64_
 
 hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &batch, const DeviceTables &tab,
                               const uint32_t *order, const std::vector<QueueSlice> &slices, uint32_t *counters,
-                              uint32_t *rescue_list, hipStream_t s) {
+                              uint32_t *rescue_list, hipStream_t s, bool packed) {
     if (slices.size() > static_cast<size_t>(kMaxSlices)) return hipErrorInvalidValue;
     for (size_t k = 0; k < slices.size(); ++k) {
         const QueueSlice &q = slices[k];
-        const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block);
+        const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block, packed);
         if (sh.waves_per_block == 0) return hipErrorInvalidValue;
         const int64_t n32 = q.count - q.n_direct;
         if (n32 == 0) continue;
@@ -490,7 +496,8 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true);
+    const bool packed = d->hap_bases && use_packed(ctx, haps_packable(d->hap_bases, d->hap_bytes));
+    int32_t vst = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true, packed);
     if (vst != PAIRHMM_OK) return vst;
     const int64_t np = d->n_pairs;
     // Counter values at the start of every pass (the rescue list's head is copied once, below).
@@ -515,6 +522,7 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
     bt->n_pairs = np;
     bt->cells = cells;
     bt->max_hap_len = max_h;
+    bt->packed = packed;
     if (hipMalloc(&bt->d_mem, off) != hipSuccess) {
         delete bt;
         return fail(PAIRHMM_ERR_OOM, "device allocation of " + std::to_string(off) + " bytes failed");
@@ -565,7 +573,7 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_OOM, "FP64 plane allocation failed");
     }
-    if (forward_shape(false, max_h, ctx->waves_per_block).waves_per_block == 0 ||
+    if (forward_shape(false, max_h, ctx->waves_per_block, packed).waves_per_block == 0 ||
         forward_shape(true, max_h, ctx->waves_per_block).waves_per_block == 0) {
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
@@ -584,7 +592,7 @@ hipError_t enqueue_forward(pairhmm_batch *bt, hipStream_t s, hipEvent_t ev[4]) {
     hipError_t e = launch_copy_counters(bt->d_counters, bt->d_counter_init, kCounterSlots, s);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[0], s);
-    e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s);
+    e = launch_fp32_slices(ctx, bt->view, tab, bt->d_order, bt->slices, bt->d_counters, bt->d_rescue, s, bt->packed);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
     e = launch_fp64_slices(ctx, bt->view, tab, bt->d_rescue, bt->slices, bt->d_counters, true, bt->d_f64, bt->f64_bytes,
@@ -1040,9 +1048,12 @@ extern "C" int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning
     if (wpb < 1 || wpb > 8) return fail(PAIRHMM_ERR_BAD_ARG, "waves_per_block must be 1..8");
     if (gp < 1 || gp > 400) return fail(PAIRHMM_ERR_BAD_ARG, "grid_percent must be 1..400");
     if (t->queue_order != 0 && t->queue_order != 1) return fail(PAIRHMM_ERR_BAD_ARG, "queue_order must be 0 or 1");
+    if (t->stripe_schedule != 0 && t->stripe_schedule != 1)
+        return fail(PAIRHMM_ERR_BAD_ARG, "stripe_schedule must be 0 or 1");
     ctx->waves_per_block = wpb;
     ctx->grid_percent = gp;
     ctx->queue_order = t->queue_order;
+    ctx->stripe_schedule = t->stripe_schedule;
     return PAIRHMM_OK;
 }
 
@@ -1052,5 +1063,6 @@ extern "C" int32_t pairhmm_get_tuning(pairhmm_context *ctx, pairhmm_tuning *t) {
     t->waves_per_block = ctx->waves_per_block;
     t->grid_percent = ctx->grid_percent;
     t->queue_order = ctx->queue_order;
+    t->stripe_schedule = ctx->stripe_schedule;
     return PAIRHMM_OK;
 }

@@ -50,6 +50,14 @@ struct WaveLds {
 template <int kCols>
 constexpr int plane_stride() { return lds_plane_stride(kCols); }
 
+// Plane slots lane 0 reads over a stripe of `steps` wave-steps (8-column blocks from slot kLdsPad),
+// rounded up to whole 64-slot chunks of the side pass; never more than kCols.
+template <int kCols>
+__device__ __forceinline__ int plane_span(int steps) {
+    const int j = round_up(kLdsPad + steps, kWave);
+    return j < kCols ? j : kCols;
+}
+
 // Two boundary planes per wave: lane 63's row (FP32: M and I in LDS, turned into lane 0's sums A
 // and B by the side pass; FP64: A and B formed by lane 63 itself, in global memory).
 constexpr int kPlanes = 2;
@@ -120,16 +128,18 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base, const DeviceBatch &b,
 //   A(c) = M(c-1) MM0 + (I(c-1) + D(c-1)) GM0 over M,   B(c) = M(c) MI0 + I(c) II0 over I.
 // Per 64-column chunk D is a linear scan (linear_scan64) plus the carry D(j0-1) DD'^(l+1); column
 // j0-1's M, I and D come from the previous chunk's lane 63 (the chunk itself is already overwritten).
+// The pass covers plane slots [0, jmax) (jmax a multiple of 64, at most kCols): the slots lane 0 of
+// the next stripe reads.  cd0 is D left of slot 0 (0 for every row but a constant one).
 template <int kCols>
 __device__ __forceinline__ void side_pass_f32(float *P0, float *P1, int lane, float MM0, float GM0, float MI0,
-                                              float II0, float MDr, float DDr) {
+                                              float II0, float MDr, float DDr, int jmax, float cd0 = 0.f) {
     const float a1 = DDr, a2 = a1 * a1, a4 = a2 * a2, a8 = a4 * a4;
     const float p_row = ipow(a1, (lane & 15) + 1);
     const float p_31 = ipow(a1, lane >= 32 ? lane - 31 : 0);
     const float p_in = ipow(a1, lane + 1);
-    float cm = 0.f, ci = 0.f, cd = 0.f;  // M, I, D of column j0 - 1
+    float cm = 0.f, ci = 0.f, cd = cd0;  // M, I, D of column j0 - 1
     __builtin_amdgcn_wave_barrier();
-    for (int j0 = 0; j0 < kCols; j0 += kWave) {
+    for (int j0 = 0; j0 < jmax; j0 += kWave) {
         const int j = j0 + lane;
         const float mj = P0[j], ij = P1[j];
         const float mprev = wave_shr1(cm, mj), iprev = wave_shr1(ci, ij);
@@ -166,6 +176,7 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
     constexpr int kStripe = kWave;
     const int n_stripes = (R + kStripe - 1) / kStripe;
     const int steps = round_up(H + kWave, kUnroll);
+    const int jmax = plane_span<kCols>(steps);
     for (int s = 0; s < n_stripes; ++s) {
         const RowTrans<T> w = row_trans<T>(s * kStripe + lane, R, rb, rq, ri, rd, rg, tab);
         const T MM = w.MM, GM = w.GM, MI = w.MI, II = w.II, MD = w.MD, DD = w.DD, pm = w.pm, px = w.px;
@@ -173,7 +184,7 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
         if (s > 0) {
             // Row 64 s (read position 64 s - 1) is the previous stripe's last row; wave-uniform.
             const RowTrans<T> up = row_trans<T>(s * kStripe - 1, R, rb, rq, ri, rd, rg, tab);
-            side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(MM), rfl(GM), rfl(MI), rfl(II), up.MD, up.DD);
+            side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(MM), rfl(GM), rfl(MI), rfl(II), up.MD, up.DD, jmax);
         }
     T m1 = 0, i1 = 0, d1 = 0, s1 = 0;  // own cell one step back (column c-1)
         T m2 = 0, s2 = 0;                  // own cell two steps back (column c-2)
@@ -423,6 +434,350 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
     }
 }
 
+// ---- packed stripes (FP32) --------------------------------------------------------------------
+//
+// The aligned schedule above gives every pair whole 64-row stripes, so a read of R rows pays for
+// 64 ceil(R/64) rows.  Packed stripes stream the rows of successive pairs through a wave's lanes
+// instead: a stripe holds the end of one pair and the start of the next.  Each pair contributes
+// R + 3 rows to the stream, in order
+//   CONST  a row whose S = I + D is 1 in every column (M = I = 0, D' = 1 from the stripe start);
+//   ZERO   the pair's row 0: M is d0 = 2^120/H in column -1 only (its prior is d0 on the marker
+//          byte staged at column -1, 0 elsewhere, times the CONST row's S = 1), so its D, the running
+//          sum of M (DD = MD = 1), is d0 from column 0 on: D(0, c) = 2^120/H as LoglessPairHMM.java:
+//          30-36 sets it;
+//   READ   rows 1..R, with the read's own transitions, except that row 1 takes MM = MI = 0 (its
+//          row above has M = I = 0 in the model, so the terms vanish; here they hold the spike) and
+//          row R takes MD = 0 (D(R, .) adds nothing to the result, and S(R, .) = I(R, .) then feeds
+//          the summing row);
+//   SUM    M(c) = (M + I)(R, c - 1) on columns 2..H+1 (prior 1 on real haplotype bytes read one
+//          column to the left, 0 on padding), I = 0, and D' with DD = 1 is the running sum of M: at
+//          column H + 2 it holds sum_c (M + I)(R, c) and stays there (M = 0 on padding).  The sum is
+//          FP32 in column order, whatever stripe position and neighbours the pair gets, so results
+//          are identical from run to run; its relative error is at most (H - 1) 2^-24 and in
+//          practice a few ulps, since a last row's terms fall off fast away from the alignment.
+// Every value stays a sub-probability times 2^120, so no product of a zeroed transition with a
+// neighbour's value can be inf or NaN.  Rows need no other case: the step loop is the aligned
+// schedule's, with per-lane haplotype pointers and transitions.  A stripe holds rows of at most two
+// pairs (lanes past the second pair's rows idle), whose haplotypes sit in two LDS buffers; it runs
+// max(H) + 66 steps, so the SUM row reaches column H + 2 and lane 63 writes every column lane 0 of
+// the next stripe needs.  Haplotype bytes 0 (padding) and 1 (the row-0 marker) are reserved; the
+// host uses the aligned schedule for batches whose haplotypes contain them.  Waves take the queue in
+// fixed chunks (see packed_fetch).
+
+constexpr uint32_t kPadByte = 0;      // haplotype bytes outside 1..H
+constexpr uint32_t kZeroRowMark = 1;  // staged at column -1
+constexpr uint32_t kNoByte = 0x1FF;   // compares equal to no byte
+
+enum RowKind : int { kRowIdle = 0, kRowConst = 1, kRowZero = 2, kRowRead = 3, kRowSum = 4 };
+
+// A pair in flight on a wave (wave-uniform).
+struct PackedPair {
+    uint32_t pair;
+    int R, H;
+    int64_t ro;  // read offset
+    int hap_n;   // the haplotype has an 'N'
+};
+
+// LDS of a packed wave: a guard (the SUM row's prior reads one byte left of buffer 0 at lane 63's
+// first step), two haplotype buffers, the two boundary planes.
+constexpr int kHapGuard = 16;
+
+template <int kCols>
+constexpr size_t packed_lds_per_wave() {
+    return kHapGuard + 2 * static_cast<size_t>(kCols) +
+           kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(float);
+}
+
+// A wave takes the queue in chunks of kPackedChunk consecutive items, each chunk by one atomic on
+// the queue head, and streams a chunk's pairs through its stripes in queue order.  A chunk always
+// starts at lane 0 of a fresh stripe and no chunk shares a stripe with another, because a pair's
+// bits depend on where its rows fall in the stripes (a row fed through the side pass rounds
+// differently from one fed by DPP): chunks are fixed by the host's queue, so every run gives the
+// same results while the waves still balance their load dynamically.
+constexpr uint32_t kPackedChunk = 16;
+
+struct ChunkCursor {
+    uint32_t next = 0, end = 0;  // items [next, end) of the wave's current chunk
+};
+
+// Takes the next pair of the wave's current chunk; with new_chunk, first takes a new chunk when the
+// current one is used up.  A pair with an empty read has L = 0 (log10 L = -inf): it is written at
+// once and, as in the aligned schedule, listed for the FP64 pass.
+__device__ bool packed_fetch(const DeviceBatch &b, const WorkSource &ws, uint32_t n_items, const RescueSink &rs,
+                             int lane, PackedPair &p, ChunkCursor &cc, bool new_chunk) {
+    for (;;) {
+        if (cc.next >= cc.end) {
+            if (!new_chunk) return false;
+            // All lanes issue the atomic (lane 0 adds 1): the fetch has no lane-divergent branch.
+            const uint32_t c = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
+            const uint64_t first = static_cast<uint64_t>(c) * kPackedChunk;
+            if (first >= n_items) return false;
+            cc.next = static_cast<uint32_t>(first);
+            cc.end = static_cast<uint32_t>(first + kPackedChunk < n_items ? first + kPackedChunk : n_items);
+        }
+        const uint32_t it = cc.next++;
+        const uint32_t pair = __builtin_amdgcn_readfirstlane(ws.items ? ws.items[it] : it);
+        int r, h;
+        if (b.pair_read) {
+            r = b.pair_read[pair];
+            h = b.pair_hap[pair];
+        } else {
+            r = static_cast<int>(pair / static_cast<uint32_t>(b.n_haps));
+            h = static_cast<int>(pair % static_cast<uint32_t>(b.n_haps));
+        }
+        p.pair = pair;
+        p.R = __builtin_amdgcn_readfirstlane(b.rd_len[r]);
+        p.H = __builtin_amdgcn_readfirstlane(b.hap_len[h]);
+        const int64_t ro = b.rd_off[r];
+        p.ro = (static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ro >> 32))) << 32) |
+               static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ro)));
+        p.hap_n = 0;
+        if (p.R > 0) return true;
+        b.out[pair] = -__builtin_inf();
+        if (rs.list != nullptr) {
+            const uint32_t slot = __builtin_amdgcn_readfirstlane(atomicAdd(rs.count, lane == 0 ? 1u : 0u));
+            rs.list[slot] = pair;
+        }
+    }
+}
+
+// Stages pair p's haplotype in LDS buffer `hb`: byte of column c at kLdsPad + c - 1, the row-0 marker
+// at column -1, padding everywhere else (the SUM row's prior needs padding up to the stripe's end).
+template <int kCols>
+__device__ void packed_stage(const DeviceBatch &b, uint8_t *hbuf, int lane, PackedPair &p) {
+    // An explicit LDS pointer: through a generic one these become FLAT stores, which count on both
+    // wait counters out of order and turn every LDS wait of the step loop into lgkmcnt(0).
+    using lds_u8w = __attribute__((address_space(3))) uint8_t;
+    lds_u8w *hb = (lds_u8w *)hbuf;  // address-space cast: C-style only
+    const int h = b.pair_read ? b.pair_hap[p.pair] : static_cast<int>(p.pair % static_cast<uint32_t>(b.n_haps));
+    const uint8_t *hap = b.hap_bases + b.hap_off[h];
+    int has_n = 0;
+    for (int j = lane; j < kCols; j += kWave) {
+        const int c = j - kLdsPad + 1;
+        uint8_t v = kPadByte;
+        if (c >= 1 && c <= p.H) {
+            v = hap[c - 1];
+            has_n |= (v == 'N');
+        } else if (c == -1) {
+            v = kZeroRowMark;
+        }
+        hb[j] = v;
+    }
+    p.hap_n = __any(has_n) ? 1 : 0;
+}
+
+// Transitions, priors and starting state of one lane's row.
+struct PackedRow {
+    RowTrans<float> w;
+    uint32_t nval;           // second byte the prior treats as a match ('N' for read rows)
+    int shift;               // haplotype column offset of the lane's prior (-1 for SUM)
+    float m1, s1, d1, s2;    // state left of the lane's first column (columns -k-1 and -k-2)
+    int kind;
+};
+
+__device__ __forceinline__ PackedRow packed_row(int kind, int idx, const PackedPair &p, const DeviceBatch &b,
+                                                const DeviceTables &tab, int lane) {
+    PackedRow x;
+    x.w = RowTrans<float>{0, 0, 0, 0, 0, 0, 0, 0, kNoByte};
+    x.nval = kNoByte;
+    x.shift = 0;
+    x.m1 = x.s1 = x.d1 = x.s2 = 0.f;
+    x.kind = kind;
+    if (kind == kRowRead) {
+        const int r = idx - 1;  // 1-based read row
+        const int64_t ro = p.ro;
+        x.w = row_trans<float>(r - 1, p.R, b.rd_bases + ro, b.rd_quals + ro, b.rd_ins + ro, b.rd_del + ro,
+                               b.rd_gcp + ro, tab);
+        if (r == 1) {
+            x.w.MM = 0.f;
+            x.w.MI = 0.f;
+        }
+        if (r == p.R) x.w.MD = 0.f;
+        x.nval = 'N';
+    } else if (kind == kRowConst) {
+        x.w.MD = 1.f;
+        x.w.DD = 1.f;
+        x.d1 = 1.f;
+        x.s1 = 1.f;
+        x.s2 = 1.f;
+    } else if (kind == kRowZero) {
+        const float d0 = static_cast<float>(__builtin_ldexp(1.0, Scale<float>::kExp) / static_cast<double>(p.H));
+        x.w.GM = 1.f;
+        x.w.MD = 1.f;
+        x.w.DD = 1.f;
+        x.w.pm = d0;
+        x.w.base = kZeroRowMark;
+        if (lane == 0) x.m1 = d0;  // column -1 lies left of lane 0's first column
+    } else if (kind == kRowSum) {
+        x.w.MM = 1.f;
+        x.w.GM = 1.f;
+        x.w.DD = 1.f;
+        x.w.px = 1.f;
+        x.w.base = kPadByte;
+        x.shift = -1;
+    }
+    return x;
+}
+
+// One stripe of steps (the aligned schedule's step, per-lane haplotype pointers and 'N' bytes).
+// Returns the lane's D' after the last step (a SUM row's last-row sum).
+template <int kCols, bool kHapHasN>
+__device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uint8_t *hap, int lane,
+                                              const PackedRow &x, int steps) {
+    const float MM = x.w.MM, GM = x.w.GM, MI = x.w.MI, II = x.w.II, MD = x.w.MD, DD = x.w.DD, pm = x.w.pm,
+                px = x.w.px;
+    const uint32_t base = x.w.base, nval = x.nval;
+    float m1 = x.m1, i1 = 0.f, d1 = x.d1, s1 = x.s1;
+    float m2 = 0.f, s2 = x.s2;
+    using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
+    lds_u8 *hp = (lds_u8 *)(hap + kLdsPad - 1 - lane + x.shift);  // address-space cast: C-style only
+    constexpr int kLag = kWave - 1;
+    float *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag;
+    int lane_w = lane;
+    asm volatile("" : "+v"(lane_w));
+    const bool is_writer = lane_w == kLag;
+    // No scalar load may be in flight at the loop head: they complete out of order, and one pending
+    // there makes every LDS wait in the loop a full lgkmcnt(0) instead of waiting for the first reads.
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    for (int t0 = 0; t0 < steps; t0 += kUnroll) {
+        float *wmb = wm + t0, *wib = wi + t0;
+        float av[kUnroll], bv[kUnroll];
+        __builtin_memcpy(av, __builtin_assume_aligned(L.bm + kLdsPad + t0, 16), sizeof(av));
+        __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
+        uint32_t hbs[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t hb = hbs[u];
+            const bool match = (hb == base) || (kHapHasN && hb == nval);
+            const float prior = match ? pm : px;
+            float a = av[u], i = bv[u];
+            dpp_upper2(a, m2, MM, s2, GM, i, m1, MI, i1, II);
+            const float m = a * prior;
+            const float d = fmaf(d1, DD, m1);
+            const float sx = fmaf(MD, d, i);
+            if (is_writer) {
+                wmb[u] = m;
+                wib[u] = i;
+            }
+            m2 = m1;
+            s2 = s1;
+            m1 = m;
+            i1 = i;
+            d1 = d;
+            s1 = sx;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return d1;
+}
+
+// The packed-stripe FP32 pass of one wave over its share of a queue.
+template <int kCols>
+__device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const WorkSource &ws, uint32_t n_items,
+                            const RescueSink &rs, uint8_t *lds, int lane) {
+    ChunkCursor cc;
+    WaveLds<float> L;
+    // Haplotype buffer k at lds + kHapGuard + k kCols (chosen by arithmetic, not by indexing an array
+    // of pointers, so the compiler keeps them LDS pointers).
+    auto hapbuf = [&](int k) { return lds + kHapGuard + (k ? kCols : 0); };
+    L.hap = hapbuf(0);
+    L.bm = reinterpret_cast<float *>(lds + kHapGuard + 2 * kCols);
+    L.bi = L.bm + plane_stride<kCols>();
+    for (int j = lane; j < plane_stride<kCols>(); j += kWave) {
+        L.bm[j] = 0.f;
+        L.bi[j] = 0.f;
+    }
+    const double scale_log10 = static_cast<double>(Scale<float>::kExp) * 0.30102999566398119521;
+    PackedPair cur{}, nxt{};
+    bool have_cur = false;
+    int cur_next = 0;  // next row of cur: 0 CONST, 1 ZERO, 2..R+1 READ, R+2 SUM
+    int buf_cur = 0;
+    bool first = true;
+    float md63 = 0.f, dd63 = 0.f, cd63 = 0.f;  // lane 63's row of the previous stripe (side pass)
+    for (;;) {
+        if (!have_cur) {
+            if (!packed_fetch(b, ws, n_items, rs, lane, cur, cc, true)) break;
+            have_cur = true;
+            cur_next = 0;
+            buf_cur = 0;
+            packed_stage<kCols>(b, hapbuf(buf_cur), lane, cur);
+        }
+        const int rows_cur = cur.R + 3 - cur_next;
+        bool two = false;
+        if (rows_cur < kWave) {
+            if (packed_fetch(b, ws, n_items, rs, lane, nxt, cc, false)) {
+                two = true;
+                packed_stage<kCols>(b, hapbuf(buf_cur ^ 1), lane, nxt);
+            }
+        }
+        // This lane's row.
+        const bool in_cur = lane < rows_cur;
+        const int idx = in_cur ? cur_next + lane : lane - rows_cur;
+        const PackedPair &pp = in_cur ? cur : nxt;
+        int kind = kRowIdle;
+        if (in_cur || (two && idx < nxt.R + 3)) {
+            const int R = in_cur ? cur.R : nxt.R;
+            kind = idx == 0 ? kRowConst : idx == 1 ? kRowZero : idx == R + 2 ? kRowSum : kRowRead;
+        }
+        const PackedRow x = packed_row(kind, idx, pp, b, tab, lane);
+        const uint8_t *hap = hapbuf(in_cur ? buf_cur : buf_cur ^ 1);
+        const int hmax = two && nxt.H > cur.H ? nxt.H : cur.H;
+        // Wave-uniform by construction; readfirstlane makes it a scalar, so the step loop stays a
+        // uniform loop (a vector trip count turns it into a divergent loop with exec masking).
+        const int steps = __builtin_amdgcn_readfirstlane(round_up(hmax + kWave + 2, kUnroll));
+        if (!first)
+            side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(x.w.MM), rfl(x.w.GM), rfl(x.w.MI), rfl(x.w.II), md63, dd63,
+                                 plane_span<kCols>(steps), cd63);
+        first = false;
+        md63 = lane63(x.w.MD);
+        dd63 = lane63(x.w.DD);
+        cd63 = lane63(x.kind == kRowConst ? 1.f : 0.f);
+        const float dfin = (cur.hap_n || (two && nxt.hap_n)) ? packed_steps<kCols, true>(L, hap, lane, x, steps)
+                                                            : packed_steps<kCols, false>(L, hap, lane, x, steps);
+        // SUM rows that ended here: every lane takes the SUM lane's total, so the result and the
+        // rescue decision are wave-uniform.  Unscaling and log10 are FP64.
+        const double total = static_cast<double>(dfin);
+        const bool cur_done = rows_cur <= kWave;
+        const bool nxt_done = two && nxt.R + 3 <= kWave - rows_cur;
+        for (int q = 0; q < 2; ++q) {
+            if (q == 0 ? !cur_done : !nxt_done) continue;
+            const int sl = q == 0 ? rows_cur - 1 : rows_cur + nxt.R + 2;
+            const uint64_t tb = __builtin_bit_cast(uint64_t, total);
+            const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(tb), sl);
+            const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(tb >> 32), sl);
+            const double sum = __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+            const uint32_t pair = q == 0 ? cur.pair : nxt.pair;
+            b.out[pair] = log10(sum) - scale_log10;
+            if (rs.list != nullptr && !(sum >= rs.threshold)) {
+                const uint32_t slot = __builtin_amdgcn_readfirstlane(atomicAdd(rs.count, lane == 0 ? 1u : 0u));
+                rs.list[slot] = pair;
+            }
+        }
+        if (!cur_done) {
+            cur_next += kWave;
+        } else if (two && !nxt_done) {
+            cur = nxt;
+            cur_next = kWave - rows_cur;
+            buf_cur ^= 1;
+        } else {
+            have_cur = false;
+        }
+    }
+}
+
+template <int kCols>
+__global__ __launch_bounds__(512) void forward_kernel_packed(DeviceBatch b, DeviceTables tab, WorkSource ws,
+                                                             RescueSink rs) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
+    packed_wave<kCols>(b, tab, ws, n_items, rs, lds_raw + packed_lds_per_wave<kCols>() * wave, lane);
+}
+
 __global__ void zero_counters_kernel(uint32_t *c, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
 }
@@ -447,8 +802,11 @@ int lds_class_index(int hap_len) {
     return -1;
 }
 
-size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
+size_t lds_bytes_per_wave(int max_hap_len, bool fp64, bool packed) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
+    if (cols == 0) return 0;
+    if (packed && !fp64)
+        return kHapGuard + 2 * cols + kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols)));
     return cols + (fp64 ? 0 : kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
 }
 
@@ -460,9 +818,10 @@ size_t fp64_plane_bytes_per_wave(int max_hap_len) {
 namespace {
 
 template <typename F>
-hipError_t with_kernel(bool fp64, int cols, F &&f) {
+hipError_t with_kernel(bool fp64, bool packed, int cols, F &&f) {
     return dispatch_col_class(cols, [&](auto C) {
-        return fp64 ? f(forward_kernel<double, decltype(C)::value>) : f(forward_kernel<float, decltype(C)::value>);
+        if (fp64) return f(forward_kernel<double, decltype(C)::value>);
+        return packed ? f(forward_kernel_packed<decltype(C)::value>) : f(forward_kernel<float, decltype(C)::value>);
     });
 }
 
@@ -473,8 +832,10 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
                           const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,
                           hipStream_t stream) {
     if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
+    // The packed schedule takes chunks from a queue: it needs a queue head.
+    if (shape.packed && (fp64 || work.head == nullptr)) return hipErrorInvalidValue;
     const dim3 block(kWave * shape.waves_per_block);
-    return with_kernel(fp64, lds_columns(max_hap_len), [&](auto kernel) {
+    return with_kernel(fp64, shape.packed, lds_columns(max_hap_len), [&](auto kernel) {
         if (shape.lds_bytes > 65536)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(shape.lds_bytes));
@@ -505,19 +866,21 @@ size_t max_lds_per_block() {
     return limit;
 }
 
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed) {
     LaunchShape sh;
     const int cols = lds_columns(max_hap_len);
     if (cols == 0) return sh;
-    const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64);
+    packed = packed && !fp64;
+    const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64, packed);
     const size_t budget = max_lds_per_block();
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
     while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > budget) --wpb;
     if (per_wave * static_cast<size_t>(wpb) > budget) return sh;
     sh.waves_per_block = wpb;
     sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
+    sh.packed = packed;
     int blocks = 0;
-    hipError_t e = with_kernel(fp64, cols, [&](auto kernel) {
+    hipError_t e = with_kernel(fp64, packed, cols, [&](auto kernel) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * wpb, sh.lds_bytes);
     });
     sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;

@@ -78,7 +78,9 @@ constexpr int lds_plane_stride(int cols) { return cols + 4; }
 
 // Bytes of LDS one wavefront needs for haplotypes up to max_hap_len.
 int lds_columns(int max_hap_len);
-size_t lds_bytes_per_wave(int max_hap_len, bool fp64);
+// LDS of one wave: FP32 aligned stripes (haplotype + two planes), FP32 packed stripes (two haplotype
+// buffers + two planes) or FP64 (haplotype; its planes live in global memory).
+size_t lds_bytes_per_wave(int max_hap_len, bool fp64, bool packed = false);
 // Index of the LDS column class that holds haplotypes of this length (0 = smallest), -1 if none.
 // Queues split by class let short haplotypes run at the occupancy their smaller LDS slice allows.
 int lds_class_index(int hap_len);
@@ -91,10 +93,12 @@ struct LaunchShape {
     int waves_per_block = 0;  // 0: the haplotype is too long for one wave's LDS slice
     int blocks_per_cu = 0;    // resident blocks per CU at this LDS size (occupancy API)
     size_t lds_bytes = 0;     // dynamic LDS per block
+    bool packed = false;      // FP32 packed-stripe schedule (rows of consecutive pairs share stripes)
 };
 
-// Largest waves-per-block <= requested whose LDS fits kMaxLdsPerBlock, and its occupancy.
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block);
+// Largest waves-per-block <= requested whose LDS fits kMaxLdsPerBlock, and its occupancy.  packed
+// selects the FP32 packed-stripe kernel (ignored for FP64), which takes its pairs from a queue.
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed = false);
 
 // Bytes of global boundary planes one FP64 wave needs for haplotypes up to max_hap_len (0 if too long).
 size_t fp64_plane_bytes_per_wave(int max_hap_len);

@@ -97,13 +97,14 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     std::vector<QueueSlice> slices;
     int64_t cells = 0;
     int max_h = 1;
-    st = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true);
+    const bool packed = d->hap_bases && use_packed(ctx, haps_packable(d->hap_bases, d->hap_bytes));
+    st = validate_and_order(d, ctx->queue_order, order, cells, max_h, slices, true, packed);
     if (st != PAIRHMM_OK) return st;
     std::vector<uint32_t> counter_init, rescue_init;
     direct_rescue_init(slices, order, counter_init, rescue_init);
     const int64_t np = d->n_pairs;
     if (np == 0) return PAIRHMM_OK;
-    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block);
+    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, packed);
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
     if (sh32.waves_per_block == 0 || sh64.waves_per_block == 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
@@ -166,7 +167,7 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     HIP_TRY(hipStreamWaitEvent(cs, sl.h2d_done, 0), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_copy_counters(cnt, reinterpret_cast<const uint32_t *>(m + o_cinit), kCounterSlots, cs),
             PAIRHMM_ERR_KERNEL);
-    HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs), PAIRHMM_ERR_KERNEL);
+    HIP_TRY(launch_fp32_slices(ctx, v, tab, ord, slices, cnt, res, cs, packed), PAIRHMM_ERR_KERNEL);
     HIP_TRY(launch_fp64_slices(ctx, v, tab, res, slices, cnt, true, static_cast<double *>(sl.f64.ptr), sl.f64.cap, cs),
             PAIRHMM_ERR_KERNEL);
     double *ho = static_cast<double *>(sl.hout.ptr);

@@ -145,6 +145,7 @@ typedef struct pairhmm_run_stats {
 int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *batch, int32_t mode, double *out_host,
                               pairhmm_run_stats *stats);
 
+/* Releases a batch.  Batches (and streams) must be released before pairhmm_done of their context. */
 int32_t pairhmm_batch_free(pairhmm_batch *batch);
 
 /* Launch schedule of the batched forward (new).  Applied by later pairhmm_batch_upload calls on
@@ -153,6 +154,10 @@ typedef struct pairhmm_tuning {
     int32_t waves_per_block; /* wavefronts per workgroup, 1..8 (0: 4)                          */
     int32_t grid_percent;    /* persistent grid as a percentage of resident capacity (0: 100)  */
     int32_t queue_order;     /* 0: pairs by descending cost (default), 1: submission order     */
+    int32_t stripe_schedule; /* FP32 pass: 0 packed stripes (rows of consecutive pairs share a
+                                wave's 64-row stripes; default), 1 aligned (one pair per stripe,
+                                the read's last stripe padded).  Batches whose haplotypes hold the
+                                bytes 0 or 1 always run aligned.                                 */
 } pairhmm_tuning;
 
 int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning *tuning);

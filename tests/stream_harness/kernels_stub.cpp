@@ -27,9 +27,10 @@ int lds_class_index(int hap_len) {
     return -1;
 }
 
-size_t lds_bytes_per_wave(int max_hap_len, bool fp64) {
+size_t lds_bytes_per_wave(int max_hap_len, bool fp64, bool packed) {
     const size_t cols = static_cast<size_t>(lds_columns(max_hap_len));
-    return cols + (fp64 ? 0 : 2 * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
+    return (packed && !fp64 ? 16 + 2 * cols : cols) +
+           (fp64 ? 0 : 2 * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
 }
 
 size_t fp64_plane_bytes_per_wave(int max_hap_len) {
@@ -39,10 +40,11 @@ size_t fp64_plane_bytes_per_wave(int max_hap_len) {
 
 size_t max_lds_per_block() { return 163840; }
 
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block) {
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed) {
     LaunchShape sh;
     if (lds_columns(max_hap_len) == 0) return sh;
-    const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64);
+    sh.packed = packed && !fp64;
+    const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64, sh.packed);
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
     while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > max_lds_per_block()) --wpb;
     sh.waves_per_block = wpb;
@@ -59,6 +61,7 @@ static void die(const char *what) {
 hipError_t launch_forward(bool fp64, const DeviceBatch &b, const DeviceTables &, const WorkSource &ws,
                           const RescueSink &, int max_hap_len, const LaunchShape &shape, int grid_blocks, hipStream_t) {
     if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
+    if (shape.packed && (fp64 || ws.head == nullptr)) die("packed launch without a queue head");
     if (fp64) {
         const size_t waves = static_cast<size_t>(grid_blocks) * shape.waves_per_block;
         if (!b.f64_planes || b.f64_plane_waves < waves) die("FP64 grid exceeds its boundary planes");

@@ -1,0 +1,108 @@
+"""The FP32 pass's two stripe schedules against the FP64 oracle.
+
+Packed stripes stream the rows of consecutive pairs through a wave's 64-row stripes (a pair's rows
+plus three transition rows); aligned stripes give every pair whole stripes.  Both must hold
+|d log10 L| <= 1e-4 on shapes that exercise the packed transitions: short reads (several pairs'
+rows in one stripe, lanes left idle past the second pair), reads ending at 61/62/63/64 rows (the
+summing row at a stripe's last lanes and the next pair's first rows at lane 0), empty reads, 'N'
+haplotype bytes, and haplotypes holding the bytes the packed schedule reserves (0 and 1), which
+run aligned whatever the tuning says.  Packed results are identical from run to run.
+"""
+import numpy as np
+import pytest
+
+from gatk_amd import _native, synth
+from gatk_amd.autotune import Schedule
+from gatk_amd.batch import BatchEngine, pack_regions
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _region(rng, read_lens, hap_len=300, n_haps=5, hap_n=False, reserved=False):
+    ref = rng.choice(list(b"ACGT"), hap_len + 20).astype(np.uint8)
+    haps = []
+    for k in range(n_haps):
+        h = ref[k:k + hap_len - 3 * k].copy()
+        if hap_n and k == 1:
+            h[10:14] = ord("N")
+        if reserved and k == 2:
+            h[5] = 1
+        haps.append(h)
+    reads = []
+    for R in read_lens:
+        st = int(rng.integers(0, max(1, hap_len - R)))
+        b = np.resize(ref[st:st + R], R).copy()
+        flip = rng.random(R) < 0.02
+        b[flip] = rng.choice(list(b"ACGT"), int(flip.sum())).astype(np.uint8)
+        reads.append((b, rng.integers(10, 41, R).astype(np.uint8), rng.integers(30, 50, R).astype(np.uint8),
+                      rng.integers(30, 50, R).astype(np.uint8), rng.integers(8, 15, R).astype(np.uint8)))
+    return haps, reads
+
+
+def _batch(seed):
+    rng = np.random.default_rng(seed)
+    regions = [
+        _region(rng, [1, 2, 3, 5, 9, 17, 30]),             # several pairs per stripe
+        _region(rng, [61, 62, 63, 64, 65, 125, 126, 128]),  # stripe-boundary endings
+        _region(rng, [0, 40, 0, 150], hap_n=True),         # empty reads, 'N' haplotype bytes
+        _region(rng, [100, 170, 250], hap_len=600, n_haps=7),
+    ]
+    return pack_regions(regions)
+
+
+@pytest.mark.parametrize("stripes", [0, 1], ids=["packed", "aligned"])
+def test_stripe_schedules_match_oracle(stripes):
+    b = _batch(3)
+    want = oracle.logless_batch(b, n_threads=8)
+    with BatchEngine() as eng:
+        Schedule(stripe_schedule=stripes).apply(eng)
+        dev = eng.upload(b)
+        outs = []
+        for _ in range(2):
+            o = np.empty(b.n_pairs)
+            dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=o)
+            outs.append(o)
+        dev.free()
+    inf = np.isinf(want)
+    assert inf.any() and np.array_equal(np.isinf(outs[0]), inf)
+    assert np.max(np.abs(outs[0][~inf] - want[~inf])) <= 1e-4
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_reserved_haplotype_bytes_run_aligned():
+    """Haplotype bytes 0 or 1 switch a batch to aligned stripes: its results equal the aligned
+    schedule's bit for bit, with the packed tuning in force."""
+    rng = np.random.default_rng(9)
+    b = pack_regions([_region(rng, [20, 90, 140], reserved=True)])
+    assert (b.hap_bases == 1).any()
+    outs = []
+    for stripes in (0, 1):
+        with BatchEngine() as eng:
+            Schedule(stripe_schedule=stripes).apply(eng)
+            dev = eng.upload(b)
+            o = np.empty(b.n_pairs)
+            dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=o)
+            dev.free()
+            outs.append(o)
+    assert np.array_equal(outs[0], outs[1])
+    want = oracle.logless_batch(b, n_threads=8)
+    assert np.max(np.abs(outs[0] - want)) <= 1e-4
+
+
+def test_packed_and_aligned_agree_on_batched_uneven():
+    b = synth.workload("batched_uneven", n_regions=30, seed=5)
+    outs = []
+    for stripes in (0, 1):
+        with BatchEngine() as eng:
+            Schedule(stripe_schedule=stripes).apply(eng)
+            dev = eng.upload(b)
+            o = np.empty(b.n_pairs)
+            dev.forward(out=o)
+            dev.free()
+            outs.append(o)
+    assert np.max(np.abs(outs[0] - outs[1])) <= 1e-5
+    rng = np.random.default_rng(1)
+    idx = np.sort(rng.choice(b.n_pairs, 2000, replace=False))
+    want = oracle.logless_batch(b, n_threads=8, pairs=idx)
+    assert np.max(np.abs(outs[0][idx] - want)) <= 1e-4
