@@ -20,11 +20,15 @@ constexpr int kMaxQual = 254;                                       // QualityUt
 constexpr int kMMTableSize = ((kMaxQual + 1) * (kMaxQual + 2)) >> 1;  // PairHMMModel.java:71
 
 struct ModelTables {
-    double qual_err[256];                 // entry 255 is unused: such bytes are rejected
+    // [q] = 10^(q/-10) (entry 255 is unused: such bytes are rejected); [256 + q] = that value / 3.0,
+    // the mismatch prior with the tristate correction (LoglessPairHMM.java:89-90), divided here in
+    // double exactly as Java does, so the kernels need no division.
+    double qual_err[512];
     std::vector<double> mm_prob;          // kMMTableSize entries, index max*(max+1)/2 + min
 
     ModelTables() : mm_prob(kMMTableSize) {
         for (int q = 0; q < 256; ++q) qual_err[q] = std::pow(10.0, static_cast<double>(q) / -10.0);
+        for (int q = 0; q < 256; ++q) qual_err[256 + q] = qual_err[q] / 3.0;
         const double tol = 8.0, step = 0.0001;
         const int jac_len = static_cast<int>(tol / step) + 1;
         std::vector<double> jac(jac_len);

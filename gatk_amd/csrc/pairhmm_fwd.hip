@@ -95,7 +95,7 @@ __device__ __forceinline__ RowTrans<T> row_trans(int p, int R, const uint8_t *__
         w.II = static_cast<T>(ec);
         w.DD = w.II;
         w.pm = static_cast<T>(1.0 - eq);
-        w.px = static_cast<T>(eq / 3.0);
+        w.px = static_cast<T>(tab.qual_err[256 + q]);  // eq / 3.0
         if (w.base == 'N') w.px = w.pm;
     } else {
         // Rows past R pass the last row down unchanged: M = D = 0 and I(r,c) = M(r-1,c) + I(r-1,c).
@@ -133,10 +133,17 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base, const DeviceBatch &b,
 template <int kCols>
 __device__ __forceinline__ void side_pass_f32(float *P0, float *P1, int lane, float MM0, float GM0, float MI0,
                                               float II0, float MDr, float DDr, int jmax, float cd0 = 0.f) {
-    const float a1 = DDr, a2 = a1 * a1, a4 = a2 * a2, a8 = a4 * a4;
-    const float p_row = ipow(a1, (lane & 15) + 1);
-    const float p_31 = ipow(a1, lane >= 32 ? lane - 31 : 0);
-    const float p_in = ipow(a1, lane + 1);
+    const float a1 = DDr, a2 = a1 * a1, a4 = a2 * a2, a8 = a4 * a4, a16 = a8 * a8, a32 = a16 * a16;
+    // DD'^((l mod 16) + 1) by the bits of l mod 16, then the row's 16-lane steps on top:
+    // p_in = DD'^(l+1), p_31 = DD'^(l-31) for l >= 32 (1 below).
+    float p_row = a1;
+    p_row = (lane & 1) ? p_row * a1 : p_row;
+    p_row = (lane & 2) ? p_row * a2 : p_row;
+    p_row = (lane & 4) ? p_row * a4 : p_row;
+    p_row = (lane & 8) ? p_row * a8 : p_row;
+    const float p_16 = (lane & 16) ? p_row * a16 : p_row;
+    const float p_in = (lane & 32) ? p_16 * a32 : p_16;
+    const float p_31 = (lane & 32) ? p_16 : 1.f;
     float cm = 0.f, ci = 0.f, cd = cd0;  // M, I, D of column j0 - 1
     __builtin_amdgcn_wave_barrier();
     for (int j0 = 0; j0 < jmax; j0 += kWave) {

@@ -34,7 +34,7 @@ struct DeviceBatch {
 
 // Qual -> probability tables in device memory (see model_tables.hpp).
 struct DeviceTables {
-    const double *qual_err;  // 256 entries
+    const double *qual_err;  // 512 entries: 10^(q/-10) at q, that value / 3 at 256 + q
     const double *mm_prob;   // triangular matchToMatch table
 };
 
