@@ -147,17 +147,17 @@ def fwd_bwd_leg(args):
     return res, ctx
 
 
-def cpu_grad_baseline(ctx, threads, n_reads=2):
-    """FP64 adjoint port (full matrices) on every pair of a few sampled reads: CPU fwd+bwd GCUPS and
-    the accuracy of the GPU's per-read gradient sums and log10 L for those reads."""
+def cpu_grad_baseline(ctx, threads, seconds=10.0):
+    """FP64 adjoint port (full matrices) on every pair of randomly sampled reads, read after read until
+    about `seconds` of wall time: CPU fwd+bwd GCUPS, and the accuracy of the GPU's per-read gradient
+    sums and log10 L for those reads."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle
 
     b = ctx["batch"]
     rng = np.random.default_rng(7)
-    reads = rng.choice(len(b.read_len), size=n_reads, replace=False)
-    pairs = np.nonzero(np.isin(b.pair_read, reads))[0]
+    order = rng.permutation(len(b.read_len))
 
     def one(p):
         r, h = int(b.pair_read[p]), int(b.pair_hap[p])
@@ -166,27 +166,53 @@ def cpu_grad_baseline(ctx, threads, n_reads=2):
         qs = [x[o:o + R].astype(np.float64) for x in ctx["q"]]
         return oracle.logless_float_grad(b.hap_bases[ho:ho + H], b.read_bases[o:o + R], *qs), R * H
 
-    t0 = time.perf_counter()
+    pairs_of = {}
+    for p, r in enumerate(b.pair_read):
+        pairs_of.setdefault(int(r), []).append(p)
+    el, cells, n_pairs, worst_l, worst_g, done = 0.0, 0, 0, 0.0, 0.0, 0
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        res = list(ex.map(one, pairs))
-    el = time.perf_counter() - t0
-    cells = sum(c for _, c in res)
-    worst_l = max(abs(ctx["out"][p] - g["log10_l"]) for p, (g, _) in zip(pairs, res))
-    worst_g = 0.0
-    for r in reads:
-        o, R = int(b.read_off[r]), int(b.read_len[r])
-        want = {k: np.zeros(R) for k in ("qb", "qi", "qd", "qc")}
-        for p, (g, _) in zip(pairs, res):
-            if b.pair_read[p] == r:
-                for k in want:
-                    want[k] += g[k]
-        a = np.concatenate([ctx["grads"][k][o:o + R] for k in want])
-        w = np.concatenate([want[k] for k in want])
-        worst_g = max(worst_g, float(np.linalg.norm(a - w) / max(np.linalg.norm(w), 1e-300)))
+        while el < seconds and done < len(order):
+            reads = order[done:done + 2 * threads]
+            done += len(reads)
+            pairs = [p for r in reads for p in pairs_of.get(int(r), [])]
+            t0 = time.perf_counter()
+            res = list(ex.map(one, pairs))
+            el += time.perf_counter() - t0
+            cells += sum(c for _, c in res)
+            n_pairs += len(pairs)
+            worst_l = max([worst_l] + [abs(ctx["out"][p] - g["log10_l"]) for p, (g, _) in zip(pairs, res)])
+            for r in reads:
+                o, R = int(b.read_off[r]), int(b.read_len[r])
+                want = {k: np.zeros(R) for k in ("qb", "qi", "qd", "qc")}
+                for p, (g, _) in zip(pairs, res):
+                    if b.pair_read[p] == r:
+                        for k in want:
+                            want[k] += g[k]
+                a = np.concatenate([ctx["grads"][k][o:o + R] for k in want])
+                w = np.concatenate([want[k] for k in want])
+                worst_g = max(worst_g, float(np.linalg.norm(a - w) / max(np.linalg.norm(w), 1e-300)))
     return {"cpu_fwd_bwd_baseline": {"value": cells / el / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
-                                     "sample": f"{len(pairs)} pairs of {n_reads} random reads, FP64 adjoint port "
+                                     "sample": f"{n_pairs} pairs of {done} random reads, FP64 adjoint port "
                                                f"(oracle/pairhmm_oracle.c, full matrices), {el:.1f} s"},
-            "max_abs_dlog10L_sample": worst_l, "grad_rel_err_sample": worst_g}
+            "max_abs_dlog10L_sample": worst_l, "grad_rel_err_sample": worst_g, "grad_checked_reads": done}
+
+
+def reference_cpu_paths() -> dict:
+    """Whether the reference's own CPU paths can run on this host: Java LoglessPairHMM needs a JVM,
+    GKL's AVX_LOGLESS_CACHING(_OMP) needs the GKL jar with its native library.  The image ships
+    neither (no network to fetch them), so both are normally reported unavailable."""
+    import glob
+    import shutil
+
+    java = shutil.which("java")
+    jars = []
+    for pat in ("/usr/share/java/gkl*.jar", "/opt/gkl/**/gkl*.jar",
+                os.path.expanduser("~/.m2/repository/com/intel/gkl/**/gkl*.jar"),
+                os.path.expanduser("~/.gradle/caches/**/com.intel.gkl/**/gkl*.jar")):
+        jars += glob.glob(pat, recursive=True)
+    return {"java_logless_caching": "available" if java else "unavailable (no java on this host)",
+            "gkl_avx_logless_caching_omp": ("available: " + jars[0]) if (java and jars) else
+            "unavailable (no GKL jar" + ("" if java else ", no JVM") + " on this host)"}
 
 
 def streaming_leg(args):
@@ -499,10 +525,17 @@ def precision_leg(args, threads):
         wall = time.perf_counter() - t0
         ste = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
         dev.free()
-    # A pair is rescued when its FP32 scaled sum L * 2^120 is below 1e-28.
-    rescued = out + 120 * np.log10(2.0) < -28.0
+    # FP64 pairs: routed by their read's qualities before any FP32 pass, or flagged by the FP32 pass
+    # (scaled sum L * 2^120 below 1e-28).  A routed pair is a false positive of the routing predicate
+    # when its FP64 result shows the FP32 pass would have kept it (L * 2^120 >= 1e-28).
+    from gatk_amd.batch import reads_routed_to_fp64
+
+    routed = reads_routed_to_fp64(b)
+    below = out + 120 * np.log10(2.0) < -28.0
+    rescued = routed | below
     r_idx, k_idx = np.nonzero(rescued)[0], np.nonzero(~rescued & np.isfinite(out))[0]
     r_cells = b.cells(r_idx)
+    routed_fp = int(np.count_nonzero(routed & ~below))
     rng = np.random.default_rng(5)
     pick = lambda idx, n: np.sort(rng.choice(idx, size=min(n, len(idx)), replace=False))  # noqa: E731
     sk, sr = pick(k_idx, args.precision_sample), pick(r_idx, args.precision_sample)
@@ -515,6 +548,9 @@ def precision_leg(args, threads):
     hist = lambda e: np.histogram(e, bins=hist_edges)[0].tolist()  # noqa: E731
     d = {"regions": args.precision_regions, "pairs": b.n_pairs, "cells": cells,
          "rescued_pairs": int(ste.n_rescued), "rescue_fraction": round(ste.n_rescued / b.n_pairs, 4),
+         "fp64_routed_pairs": int(ste.n_routed_fp64), "fp64_flagged_by_fp32_pairs": int(ste.n_rescued - ste.n_routed_fp64),
+         "routed_pairs_fp32_would_keep": routed_fp,
+         "routed_predicate_agrees_with_host": bool(int(routed.sum()) == int(ste.n_routed_fp64)),
          "rescued_cells": int(r_cells), "gcups": round(cells / (st.total_ms * 1e-3) / 1e9, 1),
          "wall_gcups": round(cells / wall / 1e9, 1),
          "fp32_kernel_ms": round(ste.fp32_kernel_ms, 1), "fp64_rescue_ms": round(ste.rescue_kernel_ms, 1),
@@ -578,7 +614,7 @@ def main() -> int:
     ap.add_argument("--readprep-sample", type=int, default=300)
     ap.add_argument("--precision-regions", type=int, default=2_000,
                     help="regions of the precision-stress leg (0 skips it)")
-    ap.add_argument("--precision-sample", type=int, default=60,
+    ap.add_argument("--precision-sample", type=int, default=10_000,
                     help="FP32-kept and rescued pairs each checked against the FP64 oracle")
     ap.add_argument("--pd-regions", type=int, default=1_000,
                     help="regions of the partially determined PairHMM leg (0 skips it)")
@@ -661,11 +697,12 @@ def main() -> int:
     # Dominant kernel: FP32 forward, HIP events on the engine stream (eager launch).
     st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER)
     achieved = cells * FLOP_PER_CELL / (st.fp32_kernel_ms * 1e-3) / 1e12
+    achieved_gcups = cells / (st.total_ms * 1e-3) / 1e9  # FP32 and FP64 kernels of one eager pass
     log(f"eager: fp32 kernel {st.fp32_kernel_ms:.1f} ms, rescue kernel {st.rescue_kernel_ms:.1f} ms")
     dev.free()
 
     # Plain per-region drop-in path (one synchronous computeLikelihoods per region).
-    plain = None
+    plain = plain_dev = None
     if args.plain_regions > 0:
         binding = HipPairHmm()
         binding.load(None)
@@ -686,7 +723,41 @@ def main() -> int:
         pt = time.perf_counter() - t0
         binding.done()
         plain = pcells / pt / 1e9
-        log(f"plain per-region drop-in: {plain:.1f} GCUPS over {len(prepared)} regions")
+        # Device time of the same per-region work: each region as its own batch (same FP32 and FP64
+        # kernels at a per-region grid), kernels timed by HIP events with inputs already resident.
+        dev_ms = 0.0
+        for k in regs:
+            rb = batch.subset_regions([k])
+            rdev = eng.upload(rb)
+            rdev.forward(mode=_native.RUN_PERSISTENT_EAGER)
+            dev_ms += rdev.forward(mode=_native.RUN_PERSISTENT_EAGER).total_ms
+            rdev.free()
+        plain_dev = pcells / (dev_ms * 1e-3) / 1e9
+        log(f"plain per-region drop-in: {plain:.1f} GCUPS wall, {plain_dev:.1f} GCUPS device time over "
+            f"{len(prepared)} regions")
+    # Config 0: the plain forward on 64 reads x 16 haplotypes (one drop-in call), best of 20 calls.
+    small = synth.workload("plain_small")
+    s_reads, s_haps = small.region(0)
+    binding = HipPairHmm()
+    binding.load(None)
+    binding.initialize(PairHMMNativeArguments())
+    rh0, hh0 = [ReadDataHolder(*r) for r in s_reads], [HaplotypeDataHolder(h) for h in s_haps]
+    o0 = np.empty(len(s_reads) * len(s_haps))
+    best = float("inf")
+    for _ in range(21):
+        t0 = time.perf_counter()
+        binding.compute_likelihoods(rh0, hh0, o0)
+        best = min(best, time.perf_counter() - t0)
+    binding.done()
+    sdev = eng.upload(small)
+    sdev.forward(mode=_native.RUN_PERSISTENT_EAGER)
+    s_ms = min(sdev.forward(mode=_native.RUN_PERSISTENT_EAGER).total_ms for _ in range(5))
+    sdev.free()
+    plain_small = {"pairs": small.n_pairs, "cells": small.cells(), "dropin_wall_gcups": round(small.cells() / best / 1e9, 2),
+                   "device_gcups": round(small.cells() / (s_ms * 1e-3) / 1e9, 2),
+                   "workload": "plain_small: 64 reads of 150 bp x 16 haplotypes of 300 bp, one computeLikelihoods call "
+                               "(wall: best of 20, host in/out); device: the same pairs as one batch, kernels only"}
+    log(f"config 0 plain_small: {plain_small}")
     eng.close()
 
     # Forward + backward (adjoint) leg: d log10 L / d(per-position Phred qb, qi, qd, qc).
@@ -726,6 +797,7 @@ def main() -> int:
         log(f"cpu GKL-class proxy: {proxy}; FP64 port on 1 thread {one['value']:.3f} GCUPS; host {cpu['host']}")
         if bwd is not None:
             bwd.update(cpu_grad_baseline(bwd_ctx, threads))
+        cpu["reference_paths"] = reference_cpu_paths()
 
     line = {
         "metric": "PairHMM GCUPS fwd & fwd+bwd on 1 MI355X at |dlog10L|<=1e-4 vs FP64",
@@ -756,7 +828,10 @@ def main() -> int:
         "smith_waterman": swr,
         "schedule": schedule.__dict__ if schedule else None,
         "plain_per_region_gcups": round(plain, 2) if plain else None,
+        "plain_per_region_device_gcups": round(plain_dev, 2) if plain else None,
         "batched_over_plain": round(gcups / plain, 2) if plain else None,
+        "batched_over_plain_device": round(achieved_gcups / plain_dev, 2) if plain else None,
+        "plain_small": plain_small,
         "max_abs_dlog10L_vs_fp64_sample": worst,
         "checked_pairs": n_checked,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -768,7 +843,30 @@ def main() -> int:
         "cpu_gkl_class_proxy": proxy,
     }
     print(json.dumps(line), flush=True)
+    _reap_children()
     return 0
+
+
+def _reap_children() -> None:
+    """Joins multiprocessing children and logs any child process still alive at exit."""
+    import multiprocessing
+
+    for p in multiprocessing.active_children():
+        p.join(timeout=10)
+    kids = []
+    try:
+        for t in os.listdir(f"/proc/{os.getpid()}/task"):
+            with open(f"/proc/{os.getpid()}/task/{t}/children") as f:
+                kids += f.read().split()
+    except OSError:
+        pass
+    for k in kids:
+        try:
+            with open(f"/proc/{k}/cmdline", "rb") as f:
+                cmd = f.read().replace(b"\0", b" ").decode(errors="replace")[:200]
+            log(f"child process {k} alive at exit: {cmd}")
+        except OSError:
+            pass
 
 
 if __name__ == "__main__":

@@ -153,6 +153,7 @@ class RunStats(ctypes.Structure):
         ("rescue_kernel_ms", ctypes.c_double),
         ("n_rescued", ctypes.c_int64),
         ("cells", ctypes.c_int64),
+        ("n_routed_fp64", ctypes.c_int64),
     ]
 
 

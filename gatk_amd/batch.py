@@ -102,6 +102,23 @@ class PackedBatch:
         return d
 
 
+def reads_routed_to_fp64(batch: PackedBatch) -> np.ndarray:
+    """Per pair: whether the engine sends it to the FP64 kernel without an FP32 pass.  The host
+    predicate of the C-ABI: the read's expected log10 emission along a perfect alignment,
+    sum_i (1-e_i) log10(1-e_i) + e_i log10(e_i/3) over its base-quality error rates, each term
+    rounded to 2^-40 fixed point and summed exactly, is below -90."""
+    import math
+
+    score_of_q = np.array([math.floor(((1 - e) * math.log10(1 - e) + e * math.log10(e / 3) if e < 1 else -1.0)
+                                      * 2.0 ** 40 + 0.5) for e in (10.0 ** (-q / 10.0) for q in range(256))],
+                          dtype=np.int64)
+    per_base = score_of_q[batch.read_quals]
+    csum = np.concatenate([[0], np.cumsum(per_base, dtype=np.int64)])
+    off = batch.read_off.astype(np.int64)
+    read_score = csum[off + batch.read_len] - csum[off]
+    return read_score[batch.pair_read] < -90 * 2 ** 40
+
+
 def pack_regions(regions: Sequence) -> PackedBatch:
     """Pack regions given as (haplotypes, reads) where each read is a 5-tuple of byte arrays
     (bases, quals, ins_gop, del_gop, gcp).  Pairs are region-major, read-major inside a region."""
@@ -152,8 +169,9 @@ class RunStats:
     total_ms: float
     fp32_kernel_ms: float
     rescue_kernel_ms: float
-    n_rescued: int
+    n_rescued: int           # pairs computed by the FP64 kernel
     cells: int
+    n_routed_fp64: int = 0   # of n_rescued: routed by their read's qualities (the rest flagged by FP32)
 
     @property
     def gcups(self) -> float:
@@ -195,7 +213,8 @@ class DeviceBatch:
             ptr = out.ctypes.data
         _raise_for(self.engine.lib.pairhmm_batch_forward(self.engine.ctx, self._h, int(mode), ptr, ctypes.byref(st)),
                    "batch_forward")
-        return RunStats(st.total_ms, st.fp32_kernel_ms, st.rescue_kernel_ms, int(st.n_rescued), int(st.cells))
+        return RunStats(st.total_ms, st.fp32_kernel_ms, st.rescue_kernel_ms, int(st.n_rescued), int(st.cells),
+                        int(st.n_routed_fp64))
 
     def free(self) -> None:
         if self._h:

@@ -666,6 +666,8 @@ extern "C" int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *bt
         if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) st.rescue_kernel_ms = ms;
     }
     st.n_rescued = (mode == PAIRHMM_RUN_FP64) ? bt->n_pairs : rescued_total(counters);
+    if (mode != PAIRHMM_RUN_FP64)
+        for (const QueueSlice &q : bt->slices) st.n_routed_fp64 += q.n_direct;
     if (stats) *stats = st;
     return PAIRHMM_OK;
 }

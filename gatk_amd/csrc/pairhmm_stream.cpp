@@ -37,6 +37,7 @@ struct pairhmm_stream {
     int next = 0;
     int64_t cells = 0;
     int64_t rescued = 0;
+    int64_t routed = 0;  // pairs routed to FP64 by their read's qualities
     bool started = false;
     std::chrono::steady_clock::time_point t0;
 };
@@ -178,6 +179,7 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     sl.user_out = out_host;
     sl.n_pairs = np;
     s->cells += cells;
+    for (const QueueSlice &q : slices) s->routed += q.n_direct;
     return PAIRHMM_OK;
 }
 
@@ -194,10 +196,12 @@ extern "C" int32_t pairhmm_stream_finish(pairhmm_stream *s, pairhmm_run_stats *s
         r.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s->t0).count();
     r.cells = s->cells;
     r.n_rescued = s->rescued;
+    r.n_routed_fp64 = s->routed;
     if (stats) *stats = r;
     s->started = false;
     s->cells = 0;
     s->rescued = 0;
+    s->routed = 0;
     return PAIRHMM_OK;
 }
 

@@ -138,6 +138,8 @@ typedef struct pairhmm_run_stats {
                                sent there directly because their read's qualities predict an
                                FP32 underflow (expected log10 emission below -90)              */
     int64_t cells;          /* sum over pairs of read length x haplotype length                */
+    int64_t n_routed_fp64;  /* of n_rescued: pairs sent to the FP64 kernel by their read's
+                               qualities without an FP32 pass (the rest were flagged by it)     */
 } pairhmm_run_stats;
 
 /* Runs the forward pass over the whole batch.  out_host (n_pairs doubles, may be NULL) receives
