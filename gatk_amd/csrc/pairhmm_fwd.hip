@@ -129,10 +129,15 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base, const DeviceBatch &b,
 // Per 64-column chunk D is a linear scan (linear_scan64) plus the carry D(j0-1) DD'^(l+1); column
 // j0-1's M, I and D come from the previous chunk's lane 63 (the chunk itself is already overwritten).
 // The pass covers plane slots [0, jmax) (jmax a multiple of 64, at most kCols): the slots lane 0 of
-// the next stripe reads.  cd0 is D left of slot 0 (0 for every row but a constant one).
+// the next stripe reads.  cd0 is D left of slot 0 (0 for every row but a constant one).  Lane 63
+// wrote slots [1, wend) in the previous stripe (column t - 63 at step t); slot 0 and slots from wend
+// on still hold what earlier passes left there and are read as M = I = 0.  Leftovers past a
+// shorter stripe's end, turned over again by every later pass, can grow to inf, and a summing row
+// that meets inf beyond its haplotype (times its prior 0) turns NaN.
 template <int kCols>
 __device__ __forceinline__ void side_pass_f32(float *P0, float *P1, int lane, float MM0, float GM0, float MI0,
-                                              float II0, float MDr, float DDr, int jmax, float cd0 = 0.f) {
+                                              float II0, float MDr, float DDr, int jmax, float cd0 = 0.f,
+                                              int wend = kCols) {
     const float a1 = DDr, a2 = a1 * a1, a4 = a2 * a2, a8 = a4 * a4, a16 = a8 * a8, a32 = a16 * a16;
     // DD'^((l mod 16) + 1) by the bits of l mod 16, then the row's 16-lane steps on top:
     // p_in = DD'^(l+1), p_31 = DD'^(l-31) for l >= 32 (1 below).
@@ -148,7 +153,8 @@ __device__ __forceinline__ void side_pass_f32(float *P0, float *P1, int lane, fl
     __builtin_amdgcn_wave_barrier();
     for (int j0 = 0; j0 < jmax; j0 += kWave) {
         const int j = j0 + lane;
-        const float mj = P0[j], ij = P1[j];
+        const bool written = j >= kLdsPad - (kWave - 1) && j < wend;
+        const float mj = written ? P0[j] : 0.f, ij = written ? P1[j] : 0.f;
         const float mprev = wave_shr1(cm, mj), iprev = wave_shr1(ci, ij);
         const float dj = fmaf(cd, p_in, linear_scan64(mprev * MDr, a1, a2, a4, a8, p_row, p_31));
         const float dprev = wave_shr1(cd, dj);
@@ -704,6 +710,7 @@ __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const
     int buf_cur = 0;
     bool first = true;
     float md63 = 0.f, dd63 = 0.f, cd63 = 0.f;  // lane 63's row of the previous stripe (side pass)
+    int wend = 0;                              // plane slots [1, wend) lane 63 wrote in that stripe
     for (;;) {
         if (!have_cur) {
             if (!packed_fetch(b, ws, n_items, rs, lane, cur, cc, true)) break;
@@ -737,8 +744,9 @@ __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const
         const int steps = __builtin_amdgcn_readfirstlane(round_up(hmax + kWave + 2, kUnroll));
         if (!first)
             side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(x.w.MM), rfl(x.w.GM), rfl(x.w.MI), rfl(x.w.II), md63, dd63,
-                                 plane_span<kCols>(steps), cd63);
+                                 plane_span<kCols>(steps), cd63, wend);
         first = false;
+        wend = kLdsPad - (kWave - 1) + steps;
         md63 = lane63(x.w.MD);
         dd63 = lane63(x.w.DD);
         cd63 = lane63(x.kind == kRowConst ? 1.f : 0.f);

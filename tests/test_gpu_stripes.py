@@ -6,7 +6,8 @@ plus three transition rows); aligned stripes give every pair whole stripes.  Bot
 rows in one stripe, lanes left idle past the second pair), reads ending at 61/62/63/64 rows (the
 summing row at a stripe's last lanes and the next pair's first rows at lane 0), empty reads, 'N'
 haplotype bytes, and haplotypes holding the bytes the packed schedule reserves (0 and 1), which
-run aligned whatever the tuning says.  Packed results are identical from run to run.
+run aligned whatever the tuning says.  Packed results are identical from run to run, whatever
+the wave processed before (the side pass reads only the plane slots the previous stripe wrote).
 """
 import numpy as np
 import pytest
@@ -106,3 +107,45 @@ def test_packed_and_aligned_agree_on_batched_uneven():
     idx = np.sort(rng.choice(b.n_pairs, 2000, replace=False))
     want = oracle.logless_batch(b, n_threads=8, pairs=idx)
     assert np.max(np.abs(outs[0][idx] - want)) <= 1e-4
+
+
+@pytest.mark.parametrize("queue_order", [0, 1])
+def test_packed_stripes_ending_at_lane_62_in_a_row(queue_order):
+    """Reads of 60 then 61 bases on one haplotype: the first pair's rows end at lane 62 of a fresh
+    stripe and every later pair's at lane 62 of the next, so each following pair has its constant
+    row at lane 63 and its zero row at lane 0 of the next stripe."""
+    rng = np.random.default_rng(11)
+    haps, reads = _region(rng, [60, 61, 61, 61, 61, 100], n_haps=1)
+    b = pack_regions([(haps, reads)])
+    want = oracle.logless_batch(b, n_threads=8)
+    with BatchEngine() as eng:
+        Schedule(stripe_schedule=0, queue_order=queue_order).apply(eng)
+        dev = eng.upload(b)
+        o = np.empty(b.n_pairs)
+        dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=o)
+        dev.free()
+    assert np.max(np.abs(o - want)) <= 1e-4
+
+
+def test_packed_results_do_not_depend_on_the_waves_history():
+    """Submission order puts pairs of very different haplotype lengths one after another, so a
+    stripe often runs longer than the one before it.  Three passes give the same bits and the same
+    rescue list size, and the FP32 pass flags about as many pairs as the aligned schedule."""
+    b = synth.workload("batched_uneven", n_regions=1000, seed=1)
+    rescued = {}
+    with BatchEngine() as eng:
+        for stripes in (0, 1):
+            Schedule(stripe_schedule=stripes, queue_order=1).apply(eng)
+            dev = eng.upload(b)
+            outs, counts = [], []
+            for _ in range(3):
+                o = np.empty(b.n_pairs)
+                counts.append(dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=o).n_rescued)
+                outs.append(o)
+            dev.free()
+            assert counts[0] == counts[1] == counts[2], (stripes, counts)
+            assert np.array_equal(outs[0].view(np.int64), outs[1].view(np.int64))
+            assert np.array_equal(outs[0].view(np.int64), outs[2].view(np.int64))
+            rescued[stripes] = counts[0]
+    print(f"rescued: packed {rescued[0]}, aligned {rescued[1]}")
+    assert abs(rescued[0] - rescued[1]) <= 0.05 * rescued[1] + 10
