@@ -592,6 +592,8 @@ def main() -> int:
     ap.add_argument("--regions", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-regions", type=int, default=100)
+    ap.add_argument("--small-calls", type=int, default=20,
+                    help="drop-in calls timed on plain_small (64 reads x 16 haplotypes); 0 skips that leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bwd-regions", type=int, default=2_000,
                     help="regions of the forward+backward leg (0 skips it)")
@@ -735,29 +737,33 @@ def main() -> int:
         plain_dev = pcells / (dev_ms * 1e-3) / 1e9
         log(f"plain per-region drop-in: {plain:.1f} GCUPS wall, {plain_dev:.1f} GCUPS device time over "
             f"{len(prepared)} regions")
-    # Config 0: the plain forward on 64 reads x 16 haplotypes (one drop-in call), best of 20 calls.
-    small = synth.workload("plain_small")
-    s_reads, s_haps = small.region(0)
-    binding = HipPairHmm()
-    binding.load(None)
-    binding.initialize(PairHMMNativeArguments())
-    rh0, hh0 = [ReadDataHolder(*r) for r in s_reads], [HaplotypeDataHolder(h) for h in s_haps]
-    o0 = np.empty(len(s_reads) * len(s_haps))
-    best = float("inf")
-    for _ in range(21):
-        t0 = time.perf_counter()
-        binding.compute_likelihoods(rh0, hh0, o0)
-        best = min(best, time.perf_counter() - t0)
-    binding.done()
-    sdev = eng.upload(small)
-    sdev.forward(mode=_native.RUN_PERSISTENT_EAGER)
-    s_ms = min(sdev.forward(mode=_native.RUN_PERSISTENT_EAGER).total_ms for _ in range(5))
-    sdev.free()
-    plain_small = {"pairs": small.n_pairs, "cells": small.cells(), "dropin_wall_gcups": round(small.cells() / best / 1e9, 2),
-                   "device_gcups": round(small.cells() / (s_ms * 1e-3) / 1e9, 2),
-                   "workload": "plain_small: 64 reads of 150 bp x 16 haplotypes of 300 bp, one computeLikelihoods call "
-                               "(wall: best of 20, host in/out); device: the same pairs as one batch, kernels only"}
-    log(f"config 0 plain_small: {plain_small}")
+    # Config 0: the plain forward on 64 reads x 16 haplotypes (one drop-in call), best of the calls.
+    plain_small = None
+    if args.small_calls > 0:
+        small = synth.workload("plain_small")
+        s_reads, s_haps = small.region(0)
+        binding = HipPairHmm()
+        binding.load(None)
+        binding.initialize(PairHMMNativeArguments())
+        rh0, hh0 = [ReadDataHolder(*r) for r in s_reads], [HaplotypeDataHolder(h) for h in s_haps]
+        o0 = np.empty(len(s_reads) * len(s_haps))
+        best = float("inf")
+        for _ in range(args.small_calls + 1):
+            t0 = time.perf_counter()
+            binding.compute_likelihoods(rh0, hh0, o0)
+            best = min(best, time.perf_counter() - t0)
+        binding.done()
+        sdev = eng.upload(small)
+        sdev.forward(mode=_native.RUN_PERSISTENT_EAGER)
+        s_ms = min(sdev.forward(mode=_native.RUN_PERSISTENT_EAGER).total_ms for _ in range(5))
+        sdev.free()
+        plain_small = {"pairs": small.n_pairs, "cells": small.cells(),
+                       "dropin_wall_gcups": round(small.cells() / best / 1e9, 2),
+                       "device_gcups": round(small.cells() / (s_ms * 1e-3) / 1e9, 2),
+                       "workload": "plain_small: 64 reads of 150 bp x 16 haplotypes of 300 bp, one computeLikelihoods "
+                                   f"call (wall: best of {args.small_calls + 1} calls, host in/out); device: the same "
+                                   "pairs as one batch, kernels only (best of 5)"}
+        log(f"config 0 plain_small: {plain_small}")
     eng.close()
 
     # Forward + backward (adjoint) leg: d log10 L / d(per-position Phred qb, qi, qd, qc).
@@ -836,7 +842,8 @@ def main() -> int:
         "checked_pairs": n_checked,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": int(HBM_BYTES_PER_CELL * cells),
-                     "kernel": "forward_kernel<float, C> over its LDS-class launches of one pass",
+                     "kernel": ("forward_kernel_packed<C>" if (schedule is None or schedule.stripe_schedule == 0)
+                               else "forward_kernel<float, C>") + " over its LDS-class launches of one pass",
                      "kernel_ms": round(st.fp32_kernel_ms, 3),
                      "flop_per_cell": FLOP_PER_CELL},
         "cpu_baseline": cpu,

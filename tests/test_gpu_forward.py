@@ -226,16 +226,27 @@ def test_batched_uneven_persistent_graph(engine):
     dev.free()
 
 
-def test_precision_stress_rescue(engine, engine64):
+def _schedule(lib, ctx, stripes):
+    import ctypes
+    t = _native.Tuning(4, 100, 0, stripes)
+    assert lib.pairhmm_set_tuning(ctx, ctypes.byref(t)) == 0
+
+
+@pytest.mark.parametrize("stripes", [0, 1], ids=["packed", "aligned"])
+def test_precision_stress_rescue(engine64, stripes):
     """STR-rich haplotypes, low base qualities, long reads: many pairs underflow FP32 and are
-    recomputed in FP64; the combined result stays within 1e-4 of the oracle."""
+    recomputed in FP64 (both stripe schedules); the combined result stays within 1e-4 of the oracle."""
     b = synth.workload("precision_stress", n_regions=2, seed=3)
     rng = np.random.default_rng(1)
     sample = np.sort(rng.choice(b.n_pairs, size=min(300, b.n_pairs), replace=False))
     want = oracle.logless_batch(b, n_threads=8, pairs=sample)
     out = np.empty(b.n_pairs)
-    st = engine.upload(b).forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
-    assert st.n_rescued > 0
+    with BatchEngine() as eng:
+        _schedule(eng.lib, eng.ctx, stripes)
+        dev = eng.upload(b)
+        st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+        dev.free()
+    assert st.n_rescued > 0 and st.n_routed_fp64 > 0
     finite = np.isfinite(want)
     assert np.array_equal(np.isfinite(out[sample]), finite)
     assert np.max(np.abs(out[sample][finite] - want[finite])) <= FP32_TOL
@@ -244,35 +255,42 @@ def test_precision_stress_rescue(engine, engine64):
     assert np.max(np.abs(out64[sample][finite] - want[finite])) <= FP64_TOL
 
 
-def test_rescued_pairs_equal_fp64_kernel(engine):
+@pytest.mark.parametrize("stripes", [0, 1], ids=["packed", "aligned"])
+def test_rescued_pairs_equal_fp64_kernel(stripes):
     """Pairs the FP32 pass hands to the FP64 kernel carry the FP64 kernel's value bit for bit (the
     per-class rescue launches over per-slice lists); pairs it keeps are FP32 values within 1e-4 of
     the FP64 kernel."""
     b = synth.workload("precision_stress", n_regions=2, seed=5)
-    dev = engine.upload(b)
-    out = np.empty(b.n_pairs)
-    st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
-    out64 = np.empty(b.n_pairs)
-    dev.forward(mode=_native.RUN_FP64, out=out64)
-    dev.free()
-    # FP64 pairs: those the FP32 pass flagged (FP32 sum L * 2^120 below 1e-28) and those the batch sent
-    # there directly because their read's expected log10 emission is below -90 (the upload's routing).
+    with BatchEngine() as eng:
+        _schedule(eng.lib, eng.ctx, stripes)
+        dev = eng.upload(b)
+        out = np.empty(b.n_pairs)
+        st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+        out64 = np.empty(b.n_pairs)
+        dev.forward(mode=_native.RUN_FP64, out=out64)
+        dev.free()
+    # FP64 pairs: those the FP32 pass flagged (FP32 sum L * 2^120 below 1e-28) and those the batch
+    # sent there directly because their read's expected log10 emission is below -90.
     rescued = (out + 120 * np.log10(2.0) < -28.0) | _predicted_direct(b)
     assert st.n_rescued == int(rescued.sum()) and 0 < st.n_rescued < b.n_pairs
     np.testing.assert_array_equal(out[rescued], out64[rescued])
     assert np.max(np.abs(out[~rescued] - out64[~rescued])) <= FP32_TOL
 
 
-def test_drop_in_routes_like_the_batch(engine):
-    """computeLikelihoods per region on STR-rich regions: pairs whose read predicts an FP32 underflow
-    go straight to the FP64 kernel here too, so they carry the batch path's FP64 values bit for bit;
-    every other pair is within the FP32 tolerance of the batch."""
+@pytest.mark.parametrize("stripes", [0, 1], ids=["packed", "aligned"])
+def test_drop_in_routes_like_the_batch(stripes):
+    """computeLikelihoods per region on STR-rich regions, under either stripe schedule: pairs whose
+    read predicts an FP32 underflow go straight to the FP64 kernel here too, so they carry the batch
+    path's FP64 values bit for bit; every other pair is within the FP32 tolerance of the batch."""
     b = synth.workload("precision_stress", n_regions=3, seed=7)
-    dev = engine.upload(b)
-    want = np.empty(b.n_pairs)
-    dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=want)
-    dev.free()
+    with BatchEngine() as eng:
+        _schedule(eng.lib, eng.ctx, stripes)
+        dev = eng.upload(b)
+        want = np.empty(b.n_pairs)
+        st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=want)
+        dev.free()
     binding = _binding(False)
+    _schedule(_native.load_library(), binding.context, stripes)
     got = np.empty(b.n_pairs)
     try:
         for k in range(len(b.region_pair_off) - 1):
@@ -286,6 +304,7 @@ def test_drop_in_routes_like_the_batch(engine):
         binding.done()
     direct = _predicted_direct(b)
     assert 0 < int(direct.sum()) < b.n_pairs
+    assert st.n_routed_fp64 == int(direct.sum())
     np.testing.assert_array_equal(got[direct], want[direct])
     assert np.max(np.abs(got - want)) <= FP32_TOL
 
