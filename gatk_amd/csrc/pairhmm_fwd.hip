@@ -305,7 +305,6 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
         __builtin_memcpy(ab, __builtin_assume_aligned(r0, 16), sizeof(ab));
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
             double *wb = w0 + 2 * t0;
-            const bool next = t0 + kUnroll < steps;
             uint32_t hbs[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
@@ -330,9 +329,12 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 i1 = i;
                 d1 = d;
                 s1 = sx;
-                if (u == kUnroll / 2 - 1 && next)
+                // The next block's halves, loaded unconditionally (the slice has slots past the last
+                // block): with a fixed count of loads in the loop, the wait for a half is a counted
+                // one instead of a full drain at the block start.
+                if (u == kUnroll / 2 - 1)
                     __builtin_memcpy(ab, __builtin_assume_aligned(r0 + 2 * (t0 + kUnroll), 16), sizeof(ab) / 2);
-                if (u == kUnroll - 1 && next)
+                if (u == kUnroll - 1)
                     __builtin_memcpy(ab + kUnroll, __builtin_assume_aligned(r0 + 2 * (t0 + kUnroll) + kUnroll, 16),
                                      sizeof(ab) / 2);
             }
