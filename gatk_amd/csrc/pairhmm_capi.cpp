@@ -120,7 +120,8 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     }
     const int64_t n_pairs = static_cast<int64_t>(n_reads) * n_haps;
     if (n_pairs > 0xFFFFFFFFll) return fail(PAIRHMM_ERR_BAD_ARG, "too many pairs in one call");
-    const bool fp64_all = ctx->use_double;
+    // A haplotype too long for the FP32 kernels' LDS planes puts the whole call in FP64.
+    const bool fp64_all = ctx->use_double || lds_columns(max_h) > kMaxFp32Cols;
     // Reads whose qualities predict an FP32 underflow: their pairs start on the rescue list and skip
     // the FP32 pass (the batch and stream APIs route them the same way).
     std::vector<uint8_t> direct(static_cast<size_t>(n_reads), 0);
@@ -215,7 +216,7 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
     for (int32_t h = 0; h < n_haps && packable; ++h) packable = haps_packable(haps[h].haplotype_bases, haps[h].length);
     const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, use_packed(ctx, packable));
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
-    if (sh64.waves_per_block == 0 || sh32.waves_per_block == 0)
+    if (sh64.waves_per_block == 0 || (!fp64_all && sh32.waves_per_block == 0))
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     const int cus = std::max(ctx->num_cus, 1);
     auto grid_for = [&](const LaunchShape &sh) {
@@ -333,13 +334,14 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         max_cost = std::max(max_cost, c);
     }
     if (np == 0) return PAIRHMM_OK;
-    if (queue_order == 1) {
-        for (int64_t p = 0; p < np; ++p) order[p] = static_cast<uint32_t>(p);
-        slices.push_back(QueueSlice{0, np, max_h});
-        return PAIRHMM_OK;
-    }
     if (lds_class_index(max_h) < 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
+    if (queue_order == 1) {
+        // Submission order, one slice; an FP64-only class sends every pair to FP64.
+        for (int64_t p = 0; p < np; ++p) order[p] = static_cast<uint32_t>(p);
+        slices.push_back(QueueSlice{0, np, max_h, lds_columns(max_h) > kMaxFp32Cols ? np : 0});
+        return PAIRHMM_OK;
+    }
     // Counting sort by descending cost, then a stable counting sort by descending LDS class.
     std::vector<uint32_t> by_cost(static_cast<size_t>(np));
     {
@@ -358,7 +360,8 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         h_in[k] = std::max(h_in[k], H);
     }
     // Classes with few pairs are merged into the next larger one: a launch per class costs a drain
-    // of the persistent grid, worth it only for a class with real work.  At most kMaxSlices runs.
+    // of the persistent grid, worth it only for a class with real work.  The last FP32 class always
+    // closes a run, so no FP32 pair is merged into an FP64-only class.  At most kMaxSlices runs.
     std::vector<int> run_of(static_cast<size_t>(top) + 1, 0);
     std::vector<int64_t> run_start;
     std::vector<int> run_class;
@@ -366,7 +369,8 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         int64_t carried = 0;
         for (int k = 0; k <= top; ++k) {
             carried += n_in[k];
-            const bool close = (k == top) || carried >= std::max<int64_t>(4096, np / 64);
+            const bool close = (k == top) || kColClasses[k] == kMaxFp32Cols ||
+                               carried >= std::max<int64_t>(4096, np / 64);
             run_of[k] = static_cast<int>(run_class.size());
             if (close && carried > 0) {
                 run_class.push_back(k);
@@ -391,15 +395,20 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         run_n[run_of[k]] += n_in[k];
         if (n_in[k] > 0) run_h[run_of[k]] = std::max(run_h[run_of[k]], h_in[k]);
     }
+    // FP64 pairs ahead of any FP32 pass: with route_direct, those whose read predicts an FP32
+    // underflow; always, those of a run whose class is FP64-only (haplotypes too long for FP32).
     std::vector<uint8_t> direct;
-    if (route_direct) {
+    const bool wide = kColClasses[run_class.back()] > kMaxFp32Cols;
+    if (route_direct || wide) {
         std::vector<uint8_t> read_direct(static_cast<size_t>(d->n_reads), 0);
-        for (int64_t r = 0; r < d->n_reads; ++r)
-            read_direct[r] = read_routes_direct(d->read_quals + d->read_off[r], d->read_len[r]);
+        if (route_direct)
+            for (int64_t r = 0; r < d->n_reads; ++r)
+                read_direct[r] = read_routes_direct(d->read_quals + d->read_off[r], d->read_len[r]);
         direct.resize(static_cast<size_t>(np));
         for (int64_t p = 0; p < np; ++p) {
-            direct[p] = read_direct[d->pair_read[p]];
-            if (direct[p]) run_direct[run_of[lds_class_index(d->hap_len[d->pair_hap[p]])]]++;
+            const int r = run_of[lds_class_index(d->hap_len[d->pair_hap[p]])];
+            direct[p] = read_direct[d->pair_read[p]] || kColClasses[run_class[r]] > kMaxFp32Cols;
+            if (direct[p]) run_direct[r]++;
         }
     }
     // Largest class first: runs in descending order of their class; within a run the FP32 queue by
@@ -435,10 +444,10 @@ hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &bat
     if (slices.size() > static_cast<size_t>(kMaxSlices)) return hipErrorInvalidValue;
     for (size_t k = 0; k < slices.size(); ++k) {
         const QueueSlice &q = slices[k];
+        const int64_t n32 = q.count - q.n_direct;
+        if (n32 == 0) continue;  // an FP64-only slice, or every pair routed to FP64
         const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block, packed);
         if (sh.waves_per_block == 0) return hipErrorInvalidValue;
-        const int64_t n32 = q.count - q.n_direct;
-        if (n32 == 0) continue;
         RescueSink rs{rescue_list + q.begin, slice_rescue_count(counters, k), kFp32RescueThreshold};
         WorkSource w{order + q.begin, static_cast<uint32_t>(n32), nullptr, slice_head(counters, k)};
         const hipError_t e = launch_forward(false, batch, tab, w, rs, q.max_hap_len, sh,
@@ -573,8 +582,9 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_OOM, "FP64 plane allocation failed");
     }
-    if (forward_shape(false, max_h, ctx->waves_per_block, packed).waves_per_block == 0 ||
-        forward_shape(true, max_h, ctx->waves_per_block).waves_per_block == 0) {
+    // FP32 slices hold haplotypes of FP32 classes only (validate_and_order), so the FP64 kernel's
+    // shape is the one to check.
+    if (forward_shape(true, max_h, ctx->waves_per_block).waves_per_block == 0) {
         pairhmm_batch_free(bt);
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     }

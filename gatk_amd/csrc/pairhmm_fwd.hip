@@ -836,9 +836,14 @@ namespace {
 
 template <typename F>
 hipError_t with_kernel(bool fp64, bool packed, int cols, F &&f) {
-    return dispatch_col_class(cols, [&](auto C) {
-        if (fp64) return f(forward_kernel<double, decltype(C)::value>);
-        return packed ? f(forward_kernel_packed<decltype(C)::value>) : f(forward_kernel<float, decltype(C)::value>);
+    return dispatch_col_class(cols, [&](auto C) -> hipError_t {
+        constexpr int kC = decltype(C)::value;
+        if (fp64) return f(forward_kernel<double, kC>);
+        if constexpr (kC > kMaxFp32Cols) {
+            return hipErrorInvalidValue;  // FP64-only class
+        } else {
+            return packed ? f(forward_kernel_packed<kC>) : f(forward_kernel<float, kC>);
+        }
     });
 }
 
@@ -886,7 +891,7 @@ size_t max_lds_per_block() {
 LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed) {
     LaunchShape sh;
     const int cols = lds_columns(max_hap_len);
-    if (cols == 0) return sh;
+    if (cols == 0 || (!fp64 && cols > kMaxFp32Cols)) return sh;
     packed = packed && !fp64;
     const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64, packed);
     const size_t budget = max_lds_per_block();

@@ -55,9 +55,13 @@ struct RescueSink {
 
 // Column capacity classes of a wave's LDS slice (haplotype bytes + boundary planes).  The class is
 // a template parameter of every kernel so each LDS access is one base register plus an immediate
-// offset.  A class holds columns -64..H+63+8 plus slack: H + 160 <= class.
-inline constexpr int kColClasses[] = {384, 512, 640, 768, 1024, 1280, 1664, 2560, 4992};
+// offset.  A class holds columns -64..H+63+8 plus slack: H + 160 <= class.  Classes up to
+// kMaxFp32Cols run the FP32 kernels (their planes are in LDS); the wider ones only the FP64 kernel,
+// whose LDS holds just the haplotype (its planes are in global memory), so haplotypes of up to
+// 32,608 bases are taken, their pairs computed in FP64 throughout.
+inline constexpr int kColClasses[] = {384, 512, 640, 768, 1024, 1280, 1664, 2560, 4992, 8192, 16384, 32768};
 inline constexpr int kNumColClasses = static_cast<int>(sizeof(kColClasses) / sizeof(kColClasses[0]));
+inline constexpr int kMaxFp32Cols = 4992;
 
 // Calls f(std::integral_constant<int, C>{}) for the class C == cols; hipErrorInvalidValue if cols
 // is not a class.  Every kernel family instantiates through this, so all share one class list.

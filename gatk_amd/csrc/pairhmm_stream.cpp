@@ -105,9 +105,9 @@ extern "C" int32_t pairhmm_stream_submit(pairhmm_stream *s, const pairhmm_batch_
     direct_rescue_init(slices, order, counter_init, rescue_init);
     const int64_t np = d->n_pairs;
     if (np == 0) return PAIRHMM_OK;
-    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, packed);
+    // FP32 slices hold haplotypes of FP32 classes only (validate_and_order): the FP64 shape decides.
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
-    if (sh32.waves_per_block == 0 || sh64.waves_per_block == 0)
+    if (sh64.waves_per_block == 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
 
     size_t off = 0;

@@ -183,6 +183,97 @@ def test_edge_cases_match_oracle():
         assert np.all(out <= 0)
 
 
+def _repeat_pairs(read_unit, ref_unit, copies):
+    """Reads and haplotypes of repeated units with the qualities of PairHMMUnitTest's big-read case
+    (base 30, insertion 40, deletion 40, gap continuation 10)."""
+    pairs = []
+    for n_read, n_ref in copies:
+        read = np.frombuffer(read_unit * n_read, np.uint8).copy()
+        R = len(read)
+        pairs.append((np.frombuffer(ref_unit * n_ref, np.uint8).copy(),
+                      (read, np.full(R, 30, np.uint8), np.full(R, 40, np.uint8), np.full(R, 40, np.uint8),
+                       np.full(R, 10, np.uint8))))
+    return pairs
+
+
+def test_really_big_reads():
+    """PairHMMUnitTest.testReallyBigReads (PairHMMUnitTest.java:420-457): a 16-base read unit and a
+    20-base haplotype unit repeated 1/2/10/20/50 and 1/2/10/20/100 times, every case with more
+    haplotype copies than read copies (reads up to 800 bases, haplotypes up to 2,000).  The
+    reference test only asks that nothing fails; here each pair, through the drop-in in FP32 (with
+    the FP64 rescue) and in FP64, also matches the oracle."""
+    copies = [(r, h) for r in (1, 2, 10, 20, 50) for h in (1, 2, 10, 20, 100) if h > r]
+    pairs = _repeat_pairs(b"ACCAAGTAGTCACCGT", b"ACCAAGTAGTCACCGTAACG", copies)
+    b = pack_regions([([hap], [read]) for hap, read in pairs])
+    want = oracle.logless_batch(b, n_threads=8)
+    assert np.all(np.isfinite(want))
+    for use_double, tol in ((False, FP32_TOL), (True, FP64_TOL)):
+        binding = _binding(use_double)
+        try:
+            got = np.empty(len(pairs))
+            for k, (hap, read) in enumerate(pairs):
+                out = np.empty(1)
+                binding.compute_likelihoods([ReadDataHolder(*read)], [HaplotypeDataHolder(hap)], out)
+                got[k] = out[0]
+        finally:
+            binding.done()
+        assert np.max(np.abs(got - want)) <= tol
+
+
+def _wide_regions(rng, hap_lens, read_lens=(100, 250, 400)):
+    regions = []
+    for H in hap_lens:
+        ref = rng.choice(list(b"ACGT"), H + 5).astype(np.uint8)
+        haps = [ref[:H], np.concatenate([ref[:H // 2], ref[H // 2 + 3:H + 3]])]
+        reads = []
+        for R in read_lens:
+            st = int(rng.integers(0, H - R))
+            bases = ref[st:st + R].copy()
+            flip = rng.random(R) < 0.02
+            bases[flip] = rng.choice(list(b"ACGT"), int(flip.sum())).astype(np.uint8)
+            reads.append((bases, rng.integers(15, 41, R).astype(np.uint8), rng.integers(35, 50, R).astype(np.uint8),
+                          rng.integers(35, 50, R).astype(np.uint8), np.full(R, 10, np.uint8)))
+        regions.append((haps, reads))
+    return regions
+
+
+def test_haplotypes_beyond_the_fp32_classes(engine):
+    """Haplotypes longer than the FP32 kernels' largest LDS class (4,832 bases) run in the FP64
+    kernel, whose LDS holds only the haplotype, up to 32,608 bases.  In a batch they sit in their
+    own slices next to FP32 ones: their pairs carry the FP64 kernel's values (1e-9 of the oracle),
+    the rest stay FP32 (1e-4).  The drop-in takes such a region in FP64 too.  32,609 bases fail
+    cleanly."""
+    rng = np.random.default_rng(17)
+    wide = _wide_regions(rng, (4900, 9000, 20000, 32608))
+    normal = _wide_regions(rng, (300, 600), read_lens=(120, 200))
+    b = pack_regions(normal + wide)
+    want = oracle.logless_batch(b, n_threads=8)
+    out = np.empty(b.n_pairs)
+    dev = engine.upload(b)
+    st = dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=out)
+    dev.free()
+    n_normal = int(b.region_pair_off[len(normal)])
+    assert st.n_routed_fp64 >= b.n_pairs - n_normal
+    assert np.max(np.abs(out[n_normal:] - want[n_normal:])) <= FP64_TOL
+    assert np.max(np.abs(out[:n_normal] - want[:n_normal])) <= FP32_TOL
+    binding = _binding(False)
+    try:
+        for k in range(len(normal), len(normal) + len(wide)):
+            reads, haps = b.region(k)
+            p0, p1 = int(b.region_pair_off[k]), int(b.region_pair_off[k + 1])
+            got = np.empty(p1 - p0)
+            binding.compute_likelihoods([ReadDataHolder(*r) for r in reads], [HaplotypeDataHolder(h) for h in haps],
+                                        got)
+            assert np.max(np.abs(got - want[p0:p1])) <= FP64_TOL
+        too_long = rng.choice(list(b"ACGT"), 32609).astype(np.uint8)
+        read = (too_long[:50].copy(), np.full(50, 30, np.uint8), np.full(50, 40, np.uint8), np.full(50, 40, np.uint8),
+                np.full(50, 10, np.uint8))
+        with pytest.raises(ValueError):
+            binding.compute_likelihoods([ReadDataHolder(*read)], [HaplotypeDataHolder(too_long)], np.empty(1))
+    finally:
+        binding.done()
+
+
 def test_bad_arguments_raise():
     binding = _binding()
     try:
