@@ -205,15 +205,15 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
         // unaligned 8-byte loads; the explicit LDS address space keeps them ds_read_u8.
         using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
         lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
-        // Lane 63 stores column c = t - 63 at step t; lane 0 reads A, B of column c = t.  One base
-        // pointer serves both, the reads 63 elements above the stores.
-        constexpr int kLag = kWave - 1;
-        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag;
+        // Every lane stores its M and I at its own column c = t - lane at step t (no one-lane exec
+        // mask in the step).  Lane 63 writes column c last, at step c + 63, so the planes end with
+        // its row; lane 0 reads A, B of column c by step c, before any lane overwrites it.
+        using lds_t = __attribute__((address_space(3))) T;
         int lane_w = lane;
         asm volatile("" : "+v"(lane_w));
-        const bool is_writer = lane_w == kLag;
+        lds_t *wm = (lds_t *)(L.bm + kLdsPad) - lane_w, *wi = (lds_t *)(L.bi + kLdsPad) - lane_w;
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            T *wmb = wm + t0, *wib = wi + t0;
+            lds_t *wmb = wm + t0, *wib = wi + t0;
             T av[kUnroll], bv[kUnroll];
             __builtin_memcpy(av, __builtin_assume_aligned(L.bm + kLdsPad + t0, 16), sizeof(av));
             __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
@@ -234,10 +234,8 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
                 // FP32's range even where DD is close to 1.
                 const T d = fma(d1, DD, m1);
                 const T sx = fma(MD, d, i);
-                if (is_writer) {
-                    wmb[u] = m;
-                    wib[u] = i;
-                }
+                wmb[u] = m;
+                wib[u] = i;
                 m2 = m1;
                 s2 = s1;
                 m1 = m;
@@ -646,16 +644,19 @@ __device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uin
     float m2 = 0.f, s2 = x.s2;
     using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
     lds_u8 *hp = (lds_u8 *)(hap + kLdsPad - 1 - lane + x.shift);  // address-space cast: C-style only
-    constexpr int kLag = kWave - 1;
-    float *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag;
+    // Every lane stores its M and I at its own column c = t - lane at step t, so the step has no
+    // one-lane exec mask (no exec writes, no branch).  Lane 63 writes column c last, at step c + 63,
+    // so the planes end with its row; lane 0 has read A and B of column c by step c, before any lane
+    // overwrites it.
+    using lds_f32 = __attribute__((address_space(3))) float;
     int lane_w = lane;
     asm volatile("" : "+v"(lane_w));
-    const bool is_writer = lane_w == kLag;
+    lds_f32 *wm = (lds_f32 *)(L.bm + kLdsPad) - lane_w, *wi = (lds_f32 *)(L.bi + kLdsPad) - lane_w;
     // No scalar load may be in flight at the loop head: they complete out of order, and one pending
     // there makes every LDS wait in the loop a full lgkmcnt(0) instead of waiting for the first reads.
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-        float *wmb = wm + t0, *wib = wi + t0;
+        lds_f32 *wmb = wm + t0, *wib = wi + t0;
         float av[kUnroll], bv[kUnroll];
         __builtin_memcpy(av, __builtin_assume_aligned(L.bm + kLdsPad + t0, 16), sizeof(av));
         __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
@@ -673,10 +674,8 @@ __device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uin
             const float m = a * prior;
             const float d = fmaf(d1, DD, m1);
             const float sx = fmaf(MD, d, i);
-            if (is_writer) {
-                wmb[u] = m;
-                wib[u] = i;
-            }
+            wmb[u] = m;
+            wib[u] = i;
             m2 = m1;
             s2 = s1;
             m1 = m;
