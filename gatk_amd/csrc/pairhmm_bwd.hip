@@ -205,7 +205,7 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
     lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
     int lane_w = lane, lane_r = lane, lane_x = lane;
     asm volatile("" : "+v"(lane_w), "+v"(lane_r), "+v"(lane_x));
-    const bool is_writer = lane_w == kLag, is_first = lane_r == 0;
+    const bool is_first = lane_r == 0;
 
     // ---- 1. forward, top down --------------------------------------------------------------
     for (int s = 0; s < n_stripes; ++s) {
@@ -220,7 +220,12 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             }
         }
         T m1 = 0, i1 = 0, d1 = 0, s1 = 0, m2 = 0, s2 = 0;
-        T *wm = L.bm + kLdsPad - kLag, *wi = L.bi + kLdsPad - kLag, *wsp = L.bs + kLdsPad - kLag;
+        // Every lane stores its M, I and S at its own column c = t - lane at step t (no one-lane exec
+        // mask in the step): lane 63 writes column c last, at step c + 63, so the planes end with its
+        // row, and lane 0 has read A, B of column c by step c, before any lane overwrites it.
+        using lds_t = __attribute__((address_space(3))) T;
+        lds_t *wm = (lds_t *)(L.bm + kLdsPad) - lane_w, *wi = (lds_t *)(L.bi + kLdsPad) - lane_w,
+              *wsp = (lds_t *)(L.bs + kLdsPad) - lane_w;
         T *ckp = ck + static_cast<size_t>(s) * g.max_tiles * kCk * kWave + lane;
         for (int t0 = 0; t0 < steps; t0 += kTile) {
             ckp[0 * kWave] = m1;  // S one step back is I + D, recomputed on restore
@@ -236,16 +241,14 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 #pragma unroll
             for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
             __builtin_amdgcn_wave_barrier();
-            T *wmb = wm + t0, *wib = wi + t0, *wsb = wsp + t0;
+            lds_t *wmb = wm + t0, *wib = wi + t0, *wsb = wsp + t0;
 #pragma unroll
             for (int u = 0; u < kTile; ++u) {
                 T m, i, d, sx, pr;
                 fwd_step<T, kHapHasN>(w, px, hbs[u], av[u], bv[u], is_first, m1, i1, d1, s1, m2, s2, m, i, d, sx, pr);
-                if (is_writer) {
-                    wmb[u] = m;
-                    wib[u] = i;
-                    wsb[u] = sx;
-                }
+                wmb[u] = m;
+                wib[u] = i;
+                wsb[u] = sx;
             }
         }
         __builtin_amdgcn_wave_barrier();
