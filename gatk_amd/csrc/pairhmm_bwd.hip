@@ -309,14 +309,19 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             }
             __builtin_amdgcn_wave_barrier();
         }
-        // Lane 63 reads X(r+1, c+1) and gI(r+1, c) of the row below; lane 0 writes X and gI of this
-        // stripe's first row for the stripe above (column c = t at step t).
+        // Lane 63 reads X(r+1, c+1) and gI(r+1, c) of the row below; lane 0 leaves X and gI of this
+        // stripe's first row for the stripe above (column c = t at step t).  Steps run in reverse, and
+        // every lane stores at its own column c = t - lane (no one-lane exec mask in the step): a
+        // column's stores come after lane 63 has read it, and lane 0's, at step c, come last.  Lane
+        // 63's X store alone would land before its own read of that column one step later; it goes
+        // one column right, onto the column it has just read.
         const T *bxr = L.bx + kLdsPad - (kLag - 1), *bgr = L.bg + kLdsPad - kLag;
-        T *bxw = L.bx + kLdsPad, *bgw = L.bg + kLdsPad;
+        using lds_t = __attribute__((address_space(3))) T;
+        lds_t *bxw = (lds_t *)(L.bx + kLdsPad) - lane_x + (lane_x == kLag ? 1 : 0);
+        lds_t *bgw = (lds_t *)(L.bg + kLdsPad) - lane_x;
         T X1 = 0, X2 = 0, gI1 = 0, gD1 = 0;
         T aMMn = 0, aGMn = 0, aMIn = 0, aIIn = 0, aMD = 0, aDD = 0, aE = 0;
         const T *ckbase = ck + static_cast<size_t>(s) * g.max_tiles * kCk * kWave + lane;
-        const bool is_x = lane_x == 0;
         // Checkpoints are read one tile ahead so the global load overlaps the current tile.
         T cur[kCk], nxt[kCk];
 #pragma unroll
@@ -364,10 +369,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
                 aMD = fma(gD, Mp, aMD);
                 aDD = fma(gD, Dp, aDD);
                 aE = fma(gM, stAW[u], aE);
-                if (is_x) {
-                    bxw[t] = X;
-                    bgw[t] = gI;
-                }
+                bxw[t] = X;
+                bgw[t] = gI;
                 X2 = X1;
                 X1 = X;
                 gI1 = gI;
