@@ -45,6 +45,7 @@ template <typename T>
 struct WaveLds {
     uint8_t *hap;      // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
     T *bm, *bi;        // [cols]: boundary planes, column c at kLdsPad + c
+    T *ring;           // FP64: LDS ring of the rows' M, S, I (kRing64 slots each)
 };
 
 template <int kCols>
@@ -61,11 +62,17 @@ __device__ __forceinline__ int plane_span(int steps) {
 // Two boundary planes per wave: lane 63's row (FP32: M and I in LDS, turned into lane 0's sums A
 // and B by the side pass; FP64: A and B formed by lane 63 itself, in global memory).
 constexpr int kPlanes = 2;
+// FP64 step loop: every lane stores its M, S and I at its own column into an LDS ring of 128 slots
+// (plus 8 for a block that runs past slot 127), from which each 8-step block forms lane 0's sums
+// of lane 63's finished columns.
+constexpr int kRing64 = 136;
+constexpr size_t kRing64Bytes = 3 * kRing64 * sizeof(double);
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() {
-    return static_cast<size_t>(kCols) +
-           (sizeof(T) == sizeof(float) ? kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T) : 0);
+    return static_cast<size_t>(kCols) + (sizeof(T) == sizeof(float)
+                                             ? kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(T)
+                                             : kRing64Bytes);
 }
 
 // Transition probabilities and priors of the row that read position p feeds (PairHMMModel.java:
@@ -113,8 +120,10 @@ __device__ __forceinline__ WaveLds<T> carve(uint8_t *base, const DeviceBatch &b,
     w.hap = base;
     if constexpr (sizeof(T) == sizeof(float)) {
         w.bm = reinterpret_cast<T *>(base + kCols);
+        w.ring = nullptr;
     } else {
         w.bm = b.f64_planes + gwave * kPlanes * static_cast<size_t>(plane_stride<kCols>());
+        w.ring = reinterpret_cast<T *>(base + kCols);
     }
     w.bi = w.bm + plane_stride<kCols>();
     return w;
@@ -252,23 +261,30 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
     return wave_sum(acc);
 }
 
-// FP64 pass.  Lane 63 forms lane 0's sums for the next stripe itself,
-//   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II',
-// from the next stripe's first-row transitions (MM', GM', MI', II'), which are wave-uniform.  After
-// the last stripe they are the pass-through row's (0, 0, 1, 1), so the B plane ends up holding
-// M(R,c) + I(R,c).  Lane 0 takes A and B as the old values of the DPP moves (wave_shr:1 without
-// bound_ctrl leaves lane 0 unwritten) with its own MM = MI = 1, so the step has no lane-0 select.
+// FP64 pass.  Lane 0's sums of the row above for the next stripe,
+//   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II'
+// over lane 63's row, with the next stripe's first-row transitions (MM', GM', MI', II'), which are
+// wave-uniform.  After the last stripe they are the pass-through row's (0, 0, 1, 1), so the B plane
+// ends up holding M(R,c) + I(R,c).  Lane 0 takes A and B as the old values of the DPP moves
+// (wave_shr:1 without bound_ctrl leaves lane 0 unwritten) with its own MM = MI = 1, so the step has
+// no lane-0 select.
+//
+// Every lane stores its M, S and I at its own column c = t - lane into an LDS ring at step t (no
+// one-lane exec mask in the step); lane 63 writes column c last, at step c + 63.  After each block
+// of 8 steps the whole wave turns lane 63's 8 columns finished in it into their A and B and stores
+// them, one 16-byte store per column (B(c), A(c+1)), with every lane storing (lanes 8-63 repeat
+// lanes 0-7).  A lane's ring slots for block t0 start at (t0 - lane) mod 128, so a block never wraps
+// and two live columns (less than 128 apart) never share a slot.
 //
 // A and B live in the wave's slice of global memory rather than LDS (an FP64 plane is 8 bytes a
 // column; in LDS the long-haplotype classes held a CU to 5-7 waves); the slice stays in L2.  They
-// are interleaved, P[2 j] = A and P[2 j + 1] = B at j = kLdsPad + c, so lane 63's two stores per step
-// share one base with immediate offsets and a block's 8 columns are 16 contiguous doubles.  Lane 0's
-// values are read through a VGPR address (an opaque zero offset keeps the compiler from turning the
-// uniform load into a scalar one, whose cache does not see vector stores) and rolled in half a block
-// ahead of use: after step 3 a block's first half of the buffer is reloaded with the next block's
-// first four columns, after step 7 the second half.  A wait for the wave's stores at each stripe start
-// orders the previous stripe's stores before these loads; all accesses come from one wave on one
-// CU, so its L1 holds no stale copy.
+// are interleaved, P[2 j] = A and P[2 j + 1] = B at j = kLdsPad + c, so a block's 8 columns are 16
+// contiguous doubles.  Lane 0's values are read through a VGPR address (an opaque zero offset keeps
+// the compiler from turning the uniform load into a scalar one, whose cache does not see vector
+// stores) and rolled in half a block ahead of use: after step 3 a block's first half of the buffer
+// is reloaded with the next block's first four columns, after step 7 the second half.  A wait for
+// the wave's stores at each stripe start orders the previous stripe's stores before these loads;
+// all accesses come from one wave on one CU, so its L1 holds no stale copy.
 template <int kCols, bool kHapHasN>
 __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                    const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
@@ -279,15 +295,16 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
     double *P = L.bm;  // A(c) at 2 (kLdsPad + c), B(c) at 2 (kLdsPad + c) + 1
     using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
     lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
+    using lds_f64 = __attribute__((address_space(3))) double;
+    lds_f64 *const rM = (lds_f64 *)L.ring, *const rS = rM + kRing64, *const rI = rS + kRing64;
     constexpr int kLag = kWave - 1;
-    // Lane 63 computes column c = t - 63 at step t: A(c+1) at w[2 (t - t0)], B(c) one double before.
-    double *w0 = P + 2 * (kLdsPad - kLag + 1);
-    int lane_w = lane, lane_r = lane, zoff;
-    asm volatile("" : "+v"(lane_w), "+v"(lane_r));
+    int lane_r = lane, zoff;
+    asm volatile("" : "+v"(lane_r));
     asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
-    const bool is_writer = lane_w == kLag;
     const bool is_first = lane_r == 0;
     const double *r0 = P + 2 * kLdsPad + zoff;
+    // Copy-out: lane handles column offset v of the block's finished columns.
+    const int v = lane & (kUnroll - 1);
     for (int s = 0; s < n_stripes; ++s) {
         RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
         const RowTrans<double> nx = row_trans<double>((s + 1) * kWave, R, rb, rq, ri, rd, rg, tab);
@@ -302,7 +319,8 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
         double ab[2 * kUnroll];
         __builtin_memcpy(ab, __builtin_assume_aligned(r0, 16), sizeof(ab));
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            double *wb = w0 + 2 * t0;
+            const int rb0 = (t0 - lane) & 127;
+            lds_f64 *wM = rM + rb0, *wS = rS + rb0, *wI = rI + rb0;
             uint32_t hbs[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
@@ -317,10 +335,9 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 const double m = a * prior;
                 const double d = fma(d1, w.DD, m1 * w.MD);
                 const double sx = i + d;
-                if (is_writer) {
-                    wb[2 * u] = fma(sx, nx.GM, m * nx.MM);
-                    wb[2 * u - 1] = fma(i, nx.II, m * nx.MI);
-                }
+                wM[u] = m;
+                wS[u] = sx;
+                wI[u] = i;
                 m2 = m1;
                 s2 = s1;
                 m1 = m;
@@ -336,6 +353,18 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                     __builtin_memcpy(ab + kUnroll, __builtin_assume_aligned(r0 + 2 * (t0 + kUnroll) + kUnroll, 16),
                                      sizeof(ab) / 2);
             }
+            __builtin_amdgcn_wave_barrier();
+            // Lane 63 finished columns t0 - 63 .. t0 - 56 in this block, at ring slots
+            // ((t0 - 63) mod 128) + v.
+            {
+                const int slot = ((t0 - kLag) & 127) + v;
+                const double mc = rM[slot], sc = rS[slot], ic = rI[slot];
+                double2 ba;
+                ba.x = fma(ic, nx.II, mc * nx.MI);  // B(c)
+                ba.y = fma(sc, nx.GM, mc * nx.MM);  // A(c + 1)
+                __builtin_memcpy(P + 2 * (kLdsPad + t0 - kLag + v) + 1, &ba, sizeof(ba));
+            }
+            __builtin_amdgcn_wave_barrier();
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -823,7 +852,7 @@ size_t lds_bytes_per_wave(int max_hap_len, bool fp64, bool packed) {
     if (cols == 0) return 0;
     if (packed && !fp64)
         return kHapGuard + 2 * cols + kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols)));
-    return cols + (fp64 ? 0 : kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
+    return cols + (fp64 ? kRing64Bytes : kPlanes * sizeof(float) * static_cast<size_t>(lds_plane_stride(static_cast<int>(cols))));
 }
 
 size_t fp64_plane_bytes_per_wave(int max_hap_len) {
