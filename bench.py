@@ -26,10 +26,10 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_CELL = 12          # M: 6, I: 3, D: 3 (LoglessPairHMM.java:51-55)
 FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published)
-# HBM bytes per cell of the forward pass: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the FP32 and
-# FP64 forward kernels on 1,000 batched_uneven regions, each scaled by its reading for a 1 GiB
-# elementwise copy (x2.0 and x1.0); the rocprof summary is in profiles/.
-HBM_BYTES_PER_CELL = 0.0081
+# HBM bytes per cell of the forward pass: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the packed
+# FP32 forward kernel's launches on 1,000 batched_uneven regions, each scaled by its reading for a
+# 1 GiB elementwise copy (x2.0 and x1.0); the rocprof summary is in profiles/.
+HBM_BYTES_PER_CELL = 0.0055
 
 
 def log(msg: str) -> None:

@@ -89,14 +89,15 @@ def _time(eng: BatchEngine, sample: PackedBatch, cands: Iterable[Schedule], repe
 
 
 def tune(batch: PackedBatch, candidates: Iterable[Schedule] = CANDIDATES, sample_regions: int = 300,
-         final_regions: int = 2_000, finalists: int = 6, repeats: int = 2,
+         final_regions: int = 10_000, finalists: int = 4, repeats: int = 2,
          cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True, verbose: bool = False) -> Schedule:
     """Pick the fastest schedule for this workload shape (see module docstring).
 
     Two stages: every candidate is timed on the first ``sample_regions`` regions, then the
-    ``finalists`` fastest are timed again on ``final_regions`` regions, where the queue's tail and
-    the FP64 rescue weigh as they do on a full batch.  The time compared is the eager pass's FP32
-    forward plus FP64 rescue (HIP events around both kernels)."""
+    ``finalists`` fastest, and the default schedule, are timed again on up to ``final_regions``
+    regions, where the queue's tail and the FP64 rescue weigh as they do on a full batch (a small
+    sample favours schedules whose advantage is only in the tail).  The time compared is the eager
+    pass's FP32 forward plus FP64 rescue (HIP events around both kernels)."""
     key = f"{device_name()}|{workload_signature(batch)}"
     if use_cache and cache_path:
         hit = _load(cache_path).get(key)
@@ -109,7 +110,10 @@ def tune(batch: PackedBatch, candidates: Iterable[Schedule] = CANDIDATES, sample
         n2 = min(final_regions, batch.n_regions)
         if n2 > n and finalists > 0:
             final = batch.subset_regions(range(n2)) if n2 < batch.n_regions else batch
-            results = _time(eng, final, [c for _, c in results[:finalists]], repeats, verbose, "stage 2")
+            final_set = [c for _, c in results[:finalists]]
+            if Schedule() not in final_set:
+                final_set.append(Schedule())
+            results = _time(eng, final, final_set, repeats, verbose, "stage 2")
             n = n2
     best_ms, best = results[0]
     if cache_path:
