@@ -530,12 +530,14 @@ constexpr size_t packed_lds_per_wave() {
            kPlanes * static_cast<size_t>(plane_stride<kCols>()) * sizeof(float);
 }
 
-// A wave takes the queue in chunks of kPackedChunk consecutive items, each chunk by one atomic on
-// the queue head, and streams a chunk's pairs through its stripes in queue order.  A chunk always
+// A wave takes the queue in chunks of ws.chunk consecutive items, each chunk by one atomic on the
+// queue head, and streams a chunk's pairs through its stripes in queue order.  A chunk always
 // starts at lane 0 of a fresh stripe and no chunk shares a stripe with another, because a pair's
 // bits depend on where its rows fall in the stripes (a row fed through the side pass rounds
-// differently from one fed by DPP): chunks are fixed by the host's queue, so every run gives the
-// same results while the waves still balance their load dynamically.
+// differently from one fed by DPP): chunks are fixed by the queue and the launch, so every run gives
+// the same results while the waves still balance their load dynamically.  The launch sets the chunk
+// (packed_chunk): at most kPackedChunk items, and few enough that every wave of the grid gets about
+// four chunks, so a small queue (one region's pairs) still spreads over the whole GPU.
 constexpr uint32_t kPackedChunk = 16;
 
 struct ChunkCursor {
@@ -552,10 +554,10 @@ __device__ bool packed_fetch(const DeviceBatch &b, const WorkSource &ws, uint32_
             if (!new_chunk) return false;
             // All lanes issue the atomic (lane 0 adds 1): the fetch has no lane-divergent branch.
             const uint32_t c = __builtin_amdgcn_readfirstlane(atomicAdd(ws.head, lane == 0 ? 1u : 0u));
-            const uint64_t first = static_cast<uint64_t>(c) * kPackedChunk;
+            const uint64_t first = static_cast<uint64_t>(c) * ws.chunk;
             if (first >= n_items) return false;
             cc.next = static_cast<uint32_t>(first);
-            cc.end = static_cast<uint32_t>(first + kPackedChunk < n_items ? first + kPackedChunk : n_items);
+            cc.end = static_cast<uint32_t>(first + ws.chunk < n_items ? first + ws.chunk : n_items);
         }
         const uint32_t it = cc.next++;
         const uint32_t pair = __builtin_amdgcn_readfirstlane(ws.items ? ws.items[it] : it);
@@ -860,6 +862,11 @@ size_t fp64_plane_bytes_per_wave(int max_hap_len) {
     return cols == 0 ? 0 : kPlanes * sizeof(double) * static_cast<size_t>(lds_plane_stride(cols));
 }
 
+uint32_t packed_chunk(int64_t n_items, int64_t waves) {
+    const int64_t c = n_items / (4 * (waves > 0 ? waves : 1));
+    return static_cast<uint32_t>(c < 1 ? 1 : c > kPackedChunk ? kPackedChunk : c);
+}
+
 namespace {
 
 template <typename F>
@@ -882,14 +889,16 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
                           const RescueSink &rescue, int max_hap_len, const LaunchShape &shape, int grid_blocks,
                           hipStream_t stream) {
     if (shape.waves_per_block < 1 || grid_blocks < 1) return hipErrorInvalidValue;
-    // The packed schedule takes chunks from a queue: it needs a queue head.
-    if (shape.packed && (fp64 || work.head == nullptr)) return hipErrorInvalidValue;
+    // The packed schedule takes chunks from a queue with a host-known item count.
+    if (shape.packed && (fp64 || work.head == nullptr || work.n_items_dev != nullptr)) return hipErrorInvalidValue;
+    WorkSource ws = work;
+    if (shape.packed) ws.chunk = packed_chunk(work.n_items, static_cast<int64_t>(grid_blocks) * shape.waves_per_block);
     const dim3 block(kWave * shape.waves_per_block);
     return with_kernel(fp64, shape.packed, lds_columns(max_hap_len), [&](auto kernel) {
         if (shape.lds_bytes > 65536)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(shape.lds_bytes));
-        hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables, work, rescue);
+        hipLaunchKernelGGL(kernel, dim3(grid_blocks), block, shape.lds_bytes, stream, batch, tables, ws, rescue);
         return hipGetLastError();
     });
 }

@@ -44,6 +44,7 @@ struct WorkSource {
     uint32_t n_items;             // item count when n_items_dev is null
     const uint32_t *n_items_dev;  // item count read on the device (e.g. the rescue count)
     uint32_t *head;               // non-null: dynamic queue (atomic head); null: grid-stride
+    uint32_t chunk = 0;           // packed FP32 schedule: items per queue chunk (set at launch)
 };
 
 // FP32 launches append pairs whose scaled last-row sum is below `threshold` to `list`.
@@ -85,6 +86,8 @@ int lds_columns(int max_hap_len);
 // LDS of one wave: FP32 aligned stripes (haplotype + two planes), FP32 packed stripes (two haplotype
 // buffers + two planes) or FP64 (haplotype; its planes live in global memory).
 size_t lds_bytes_per_wave(int max_hap_len, bool fp64, bool packed = false);
+// Items per queue chunk of a packed FP32 launch of `waves` waves over n_items items (1..16).
+uint32_t packed_chunk(int64_t n_items, int64_t waves);
 // Index of the LDS column class that holds haplotypes of this length (0 = smallest), -1 if none.
 // Queues split by class let short haplotypes run at the occupancy their smaller LDS slice allows.
 int lds_class_index(int hap_len);
