@@ -84,7 +84,9 @@ int32_t pairhmm_initialize(const pairhmm_native_arguments *args, pairhmm_context
  * likelihoods[r * n_haplotypes + h] = log10 P(read r | haplotype h) (read-major, the layout
  * VectorLoglessPairHMM.java:131,141-152 reads).  Synchronous: returns after the results are in
  * host memory.  FP32 cells with FP64 recompute of pairs whose FP32 sum falls under the underflow
- * threshold, or FP64 throughout when use_double_precision was set.  Values are <= 0 or -inf. */
+ * threshold, or FP64 throughout when use_double_precision was set or a haplotype is longer than
+ * 4,832 bases.  Haplotypes longer than 32,608 bases are PAIRHMM_ERR_BAD_ARG.  Values are <= 0 or
+ * -inf. */
 int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
                                     int32_t n_reads, const pairhmm_haplotype_data_holder *haplotypes,
                                     int32_t n_haplotypes, double *likelihoods);
@@ -136,10 +138,11 @@ typedef struct pairhmm_run_stats {
     double rescue_kernel_ms;/* duration of the FP64 rescue kernel (eager mode only)            */
     int64_t n_rescued;      /* pairs computed by the FP64 kernel: flagged by the FP32 pass, or
                                sent there directly because their read's qualities predict an
-                               FP32 underflow (expected log10 emission below -90)              */
+                               FP32 underflow (expected log10 emission below -90) or their
+                               haplotype is too long for the FP32 kernels (over 4,832 bases)    */
     int64_t cells;          /* sum over pairs of read length x haplotype length                */
-    int64_t n_routed_fp64;  /* of n_rescued: pairs sent to the FP64 kernel by their read's
-                               qualities without an FP32 pass (the rest were flagged by it)     */
+    int64_t n_routed_fp64;  /* of n_rescued: pairs sent to the FP64 kernel without an FP32 pass
+                               (the rest were flagged by it)                                    */
 } pairhmm_run_stats;
 
 /* Runs the forward pass over the whole batch.  out_host (n_pairs doubles, may be NULL) receives
@@ -150,8 +153,9 @@ int32_t pairhmm_batch_forward(pairhmm_context *ctx, pairhmm_batch *batch, int32_
 /* Releases a batch.  Batches (and streams) must be released before pairhmm_done of their context. */
 int32_t pairhmm_batch_free(pairhmm_batch *batch);
 
-/* Launch schedule of the batched forward (new).  Applied by later pairhmm_batch_upload calls on
- * the context; gatk_amd/autotune.py searches it per device and workload. */
+/* Launch schedule of the forward (new).  Applied by later pairhmm_batch_upload, stream and
+ * computeLikelihoods calls on the context; gatk_amd/autotune.py searches it per device and
+ * workload. */
 typedef struct pairhmm_tuning {
     int32_t waves_per_block; /* wavefronts per workgroup, 1..8 (0: 4)                          */
     int32_t grid_percent;    /* persistent grid as a percentage of resident capacity (0: 100)  */
