@@ -143,7 +143,8 @@ class GradDesc(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("grid_percent", ctypes.c_int32),
-                ("queue_order", ctypes.c_int32), ("stripe_schedule", ctypes.c_int32)]
+                ("queue_order", ctypes.c_int32), ("stripe_schedule", ctypes.c_int32),
+                ("step_unroll", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):

@@ -1,11 +1,12 @@
 """Autotuner of the batched forward launch schedule.
 
-The persistent forward kernel has four schedule knobs (``pairhmm_tuning`` in
+The persistent forward kernel has five schedule knobs (``pairhmm_tuning`` in
 include/gatk_pairhmm.h): wavefronts per workgroup (which sets how the per-wave LDS slices pack into
 a CU's 160 KB), the persistent grid as a fraction of resident capacity, the work-queue order
-(descending cost vs submission order), and the stripe schedule of the FP32 kernel: packed stripes
+(descending cost vs submission order), the stripe schedule of the FP32 kernel: packed stripes
 (the rows of consecutive pairs share a wave's 64-row stripes) or aligned stripes (one pair per
-stripe, the read's last stripe padded to 64 rows).  ``tune`` uploads a sample of the workload, times
+stripe, the read's last stripe padded to 64 rows), and the packed kernel's loop body (8 or 16
+wave-steps per unrolled block, or per LDS class by occupancy).  ``tune`` uploads a sample of the workload, times
 every candidate's eager pass (FP32 forward plus FP64 rescue, HIP events, best of ``repeats``),
 re-times the fastest few on a larger sample, and returns the fastest.  Results are cached in a JSON
 file keyed by device and a coarse workload signature (haplotype / read length classes), so later
@@ -35,14 +36,18 @@ class Schedule:
     grid_percent: int = 100
     queue_order: int = 0
     stripe_schedule: int = 0  # 0 packed stripes, 1 aligned stripes
+    step_unroll: int = 0  # packed stripes: 8 or 16 wave-steps per loop body, 0 per LDS class
 
     def apply(self, engine: BatchEngine) -> None:
-        t = _native.Tuning(self.waves_per_block, self.grid_percent, self.queue_order, self.stripe_schedule)
+        t = _native.Tuning(self.waves_per_block, self.grid_percent, self.queue_order, self.stripe_schedule,
+                           self.step_unroll)
         _raise_for(engine.lib.pairhmm_set_tuning(engine.ctx, ctypes.byref(t)), "set_tuning")
 
 
-CANDIDATES: List[Schedule] = [Schedule(w, g, o, s) for w, g, o, s in
-                              itertools.product((1, 2, 4, 8), (50, 100, 200), (0, 1), (0, 1))]
+# The loop body matters only to the packed kernel, so aligned candidates keep the default.
+CANDIDATES: List[Schedule] = [Schedule(w, g, o, s, u) for w, g, o, s, u in
+                              itertools.product((1, 2, 4, 8), (50, 100, 200), (0, 1), (0, 1), (0, 8, 16))
+                              if s == 0 or u == 0]
 
 
 def workload_signature(batch: PackedBatch) -> str:

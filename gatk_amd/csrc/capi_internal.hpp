@@ -192,6 +192,7 @@ struct pairhmm_context {
     int grid_percent = 100;
     int queue_order = 0;
     int stripe_schedule = 0;  // 0 packed stripes, 1 aligned stripes (FP32 pass)
+    int step_unroll = 0;      // packed kernel's loop body: 8 or 16 wave-steps, 0 per LDS class
 };
 
 namespace gatk_pairhmm {

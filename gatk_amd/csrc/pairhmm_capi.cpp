@@ -214,7 +214,7 @@ extern "C" int32_t pairhmm_compute_likelihoods(pairhmm_context *ctx, const pairh
 
     bool packable = true;
     for (int32_t h = 0; h < n_haps && packable; ++h) packable = haps_packable(haps[h].haplotype_bases, haps[h].length);
-    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, use_packed(ctx, packable));
+    const LaunchShape sh32 = forward_shape(false, max_h, ctx->waves_per_block, use_packed(ctx, packable), ctx->step_unroll);
     const LaunchShape sh64 = forward_shape(true, max_h, ctx->waves_per_block);
     if (sh64.waves_per_block == 0 || (!fp64_all && sh32.waves_per_block == 0))
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
@@ -446,7 +446,7 @@ hipError_t launch_fp32_slices(const pairhmm_context *ctx, const DeviceBatch &bat
         const QueueSlice &q = slices[k];
         const int64_t n32 = q.count - q.n_direct;
         if (n32 == 0) continue;  // an FP64-only slice, or every pair routed to FP64
-        const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block, packed);
+        const LaunchShape sh = forward_shape(false, q.max_hap_len, ctx->waves_per_block, packed, ctx->step_unroll);
         if (sh.waves_per_block == 0) return hipErrorInvalidValue;
         RescueSink rs{rescue_list + q.begin, slice_rescue_count(counters, k), kFp32RescueThreshold};
         WorkSource w{order + q.begin, static_cast<uint32_t>(n32), nullptr, slice_head(counters, k)};
@@ -1062,10 +1062,13 @@ extern "C" int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning
     if (t->queue_order != 0 && t->queue_order != 1) return fail(PAIRHMM_ERR_BAD_ARG, "queue_order must be 0 or 1");
     if (t->stripe_schedule != 0 && t->stripe_schedule != 1)
         return fail(PAIRHMM_ERR_BAD_ARG, "stripe_schedule must be 0 or 1");
+    if (t->step_unroll != 0 && t->step_unroll != 8 && t->step_unroll != 16)
+        return fail(PAIRHMM_ERR_BAD_ARG, "step_unroll must be 0, 8 or 16");
     ctx->waves_per_block = wpb;
     ctx->grid_percent = gp;
     ctx->queue_order = t->queue_order;
     ctx->stripe_schedule = t->stripe_schedule;
+    ctx->step_unroll = t->step_unroll;
     return PAIRHMM_OK;
 }
 
@@ -1076,5 +1079,6 @@ extern "C" int32_t pairhmm_get_tuning(pairhmm_context *ctx, pairhmm_tuning *t) {
     t->grid_percent = ctx->grid_percent;
     t->queue_order = ctx->queue_order;
     t->stripe_schedule = ctx->stripe_schedule;
+    t->step_unroll = ctx->step_unroll;
     return PAIRHMM_OK;
 }

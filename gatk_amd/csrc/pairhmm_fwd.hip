@@ -665,7 +665,7 @@ __device__ __forceinline__ PackedRow packed_row(int kind, int idx, const PackedP
 
 // One stripe of steps (the aligned schedule's step, per-lane haplotype pointers and 'N' bytes).
 // Returns the lane's D' after the last step (a SUM row's last-row sum).
-template <int kCols, bool kHapHasN>
+template <int kCols, bool kHapHasN, int kU>
 __device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uint8_t *hap, int lane,
                                               const PackedRow &x, int steps) {
     const float MM = x.w.MM, GM = x.w.GM, MI = x.w.MI, II = x.w.II, MD = x.w.MD, DD = x.w.DD, pm = x.w.pm,
@@ -686,17 +686,17 @@ __device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uin
     // No scalar load may be in flight at the loop head: they complete out of order, and one pending
     // there makes every LDS wait in the loop a full lgkmcnt(0) instead of waiting for the first reads.
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-    for (int t0 = 0; t0 < steps; t0 += kUnroll) {
+    for (int t0 = 0; t0 < steps; t0 += kU) {
         lds_f32 *wmb = wm + t0, *wib = wi + t0;
-        float av[kUnroll], bv[kUnroll];
+        float av[kU], bv[kU];
         __builtin_memcpy(av, __builtin_assume_aligned(L.bm + kLdsPad + t0, 16), sizeof(av));
         __builtin_memcpy(bv, __builtin_assume_aligned(L.bi + kLdsPad + t0, 16), sizeof(bv));
-        uint32_t hbs[kUnroll];
+        uint32_t hbs[kU];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
+        for (int u = 0; u < kU; ++u) hbs[u] = hp[t0 + u];
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < kU; ++u) {
             const uint32_t hb = hbs[u];
             const bool match = (hb == base) || (kHapHasN && hb == nval);
             const float prior = match ? pm : px;
@@ -719,8 +719,9 @@ __device__ __forceinline__ float packed_steps(const WaveLds<float> &L, const uin
     return d1;
 }
 
-// The packed-stripe FP32 pass of one wave over its share of a queue.
-template <int kCols>
+// The packed-stripe FP32 pass of one wave over its share of a queue, kU wave-steps per unrolled
+// loop body.
+template <int kCols, int kU>
 __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const WorkSource &ws, uint32_t n_items,
                             const RescueSink &rs, uint8_t *lds, int lane) {
     ChunkCursor cc;
@@ -773,7 +774,7 @@ __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const
         const int hmax = two && nxt.H > cur.H ? nxt.H : cur.H;
         // Wave-uniform by construction; readfirstlane makes it a scalar, so the step loop stays a
         // uniform loop (a vector trip count turns it into a divergent loop with exec masking).
-        const int steps = __builtin_amdgcn_readfirstlane(round_up(hmax + kWave + 2, kUnroll));
+        const int steps = __builtin_amdgcn_readfirstlane(round_up(hmax + kWave + 2, kU));
         if (!first)
             side_pass_f32<kCols>(L.bm, L.bi, lane, rfl(x.w.MM), rfl(x.w.GM), rfl(x.w.MI), rfl(x.w.II), md63, dd63,
                                  plane_span<kCols>(steps), cd63, wend);
@@ -782,8 +783,8 @@ __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const
         md63 = lane63(x.w.MD);
         dd63 = lane63(x.w.DD);
         cd63 = lane63(x.kind == kRowConst ? 1.f : 0.f);
-        const float dfin = (cur.hap_n || (two && nxt.hap_n)) ? packed_steps<kCols, true>(L, hap, lane, x, steps)
-                                                            : packed_steps<kCols, false>(L, hap, lane, x, steps);
+        const float dfin = (cur.hap_n || (two && nxt.hap_n)) ? packed_steps<kCols, true, kU>(L, hap, lane, x, steps)
+                                                            : packed_steps<kCols, false, kU>(L, hap, lane, x, steps);
         // SUM rows that ended here: every lane takes the SUM lane's total, so the result and the
         // rescue decision are wave-uniform.  Unscaling and log10 are FP64.
         const double total = static_cast<double>(dfin);
@@ -815,14 +816,14 @@ __device__ void packed_wave(const DeviceBatch &b, const DeviceTables &tab, const
     }
 }
 
-template <int kCols>
+template <int kCols, int kU>
 __global__ __launch_bounds__(512) void forward_kernel_packed(DeviceBatch b, DeviceTables tab, WorkSource ws,
                                                              RescueSink rs) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const uint32_t n_items = ws.n_items_dev ? *ws.n_items_dev : ws.n_items;
-    packed_wave<kCols>(b, tab, ws, n_items, rs, lds_raw + packed_lds_per_wave<kCols>() * wave, lane);
+    packed_wave<kCols, kU>(b, tab, ws, n_items, rs, lds_raw + packed_lds_per_wave<kCols>() * wave, lane);
 }
 
 __global__ void zero_counters_kernel(uint32_t *c, int n) {
@@ -870,14 +871,15 @@ uint32_t packed_chunk(int64_t n_items, int64_t waves) {
 namespace {
 
 template <typename F>
-hipError_t with_kernel(bool fp64, bool packed, int cols, F &&f) {
+hipError_t with_kernel(bool fp64, bool packed, int cols, int unroll, F &&f) {
     return dispatch_col_class(cols, [&](auto C) -> hipError_t {
         constexpr int kC = decltype(C)::value;
         if (fp64) return f(forward_kernel<double, kC>);
         if constexpr (kC > kMaxFp32Cols) {
             return hipErrorInvalidValue;  // FP64-only class
         } else {
-            return packed ? f(forward_kernel_packed<kC>) : f(forward_kernel<float, kC>);
+            if (!packed) return f(forward_kernel<float, kC>);
+            return unroll == 16 ? f(forward_kernel_packed<kC, 16>) : f(forward_kernel_packed<kC, 8>);
         }
     });
 }
@@ -894,7 +896,7 @@ hipError_t launch_forward(bool fp64, const DeviceBatch &batch, const DeviceTable
     WorkSource ws = work;
     if (shape.packed) ws.chunk = packed_chunk(work.n_items, static_cast<int64_t>(grid_blocks) * shape.waves_per_block);
     const dim3 block(kWave * shape.waves_per_block);
-    return with_kernel(fp64, shape.packed, lds_columns(max_hap_len), [&](auto kernel) {
+    return with_kernel(fp64, shape.packed, lds_columns(max_hap_len), shape.unroll, [&](auto kernel) {
         if (shape.lds_bytes > 65536)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(shape.lds_bytes));
@@ -925,7 +927,7 @@ size_t max_lds_per_block() {
     return limit;
 }
 
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed) {
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed, int unroll) {
     LaunchShape sh;
     const int cols = lds_columns(max_hap_len);
     if (cols == 0 || (!fp64 && cols > kMaxFp32Cols)) return sh;
@@ -938,11 +940,24 @@ LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_bl
     sh.waves_per_block = wpb;
     sh.lds_bytes = per_wave * static_cast<size_t>(wpb);
     sh.packed = packed;
-    int blocks = 0;
-    hipError_t e = with_kernel(fp64, packed, cols, [&](auto kernel) {
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * wpb, sh.lds_bytes);
-    });
-    sh.blocks_per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
+    auto occupancy = [&](int u) {
+        int blocks = 0;
+        const hipError_t e = with_kernel(fp64, packed, cols, u, [&](auto kernel) {
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * wpb, sh.lds_bytes);
+        });
+        return (e == hipSuccess && blocks > 0) ? blocks : 1;
+    };
+    sh.unroll = packed && unroll == 16 ? 16 : 8;
+    sh.blocks_per_cu = occupancy(sh.unroll);
+    if (packed && unroll == 0) {
+        // Automatic: the 16-step body issues fewer instructions per step but needs more VGPRs; it
+        // wins where the LDS slice, not the registers, limits the waves per CU (long haplotypes).
+        const int b16 = occupancy(16);
+        if (b16 >= sh.blocks_per_cu) {
+            sh.unroll = 16;
+            sh.blocks_per_cu = b16;
+        }
+    }
     return sh;
 }
 

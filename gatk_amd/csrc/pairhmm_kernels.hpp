@@ -101,11 +101,14 @@ struct LaunchShape {
     int blocks_per_cu = 0;    // resident blocks per CU at this LDS size (occupancy API)
     size_t lds_bytes = 0;     // dynamic LDS per block
     bool packed = false;      // FP32 packed-stripe schedule (rows of consecutive pairs share stripes)
+    int unroll = 8;           // packed schedule: wave-steps per unrolled loop body (8 or 16)
 };
 
 // Largest waves-per-block <= requested whose LDS fits kMaxLdsPerBlock, and its occupancy.  packed
-// selects the FP32 packed-stripe kernel (ignored for FP64), which takes its pairs from a queue.
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed = false);
+// selects the FP32 packed-stripe kernel (ignored for FP64), which takes its pairs from a queue;
+// unroll picks its loop body: 8 or 16 wave-steps, or 0 for 16 where that keeps the occupancy of 8.
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed = false,
+                          int unroll = 0);
 
 // Bytes of global boundary planes one FP64 wave needs for haplotypes up to max_hap_len (0 if too long).
 size_t fp64_plane_bytes_per_wave(int max_hap_len);

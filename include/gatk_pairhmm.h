@@ -164,6 +164,9 @@ typedef struct pairhmm_tuning {
                                 wave's 64-row stripes; default), 1 aligned (one pair per stripe,
                                 the read's last stripe padded).  Batches whose haplotypes hold the
                                 bytes 0 or 1 always run aligned.                                 */
+    int32_t step_unroll;     /* packed FP32 kernel: wave-steps per unrolled loop body, 8 or 16;
+                                0 (default): per LDS class, 16 where its extra registers cost no
+                                occupancy (long haplotypes), else 8                              */
 } pairhmm_tuning;
 
 int32_t pairhmm_set_tuning(pairhmm_context *ctx, const pairhmm_tuning *tuning);

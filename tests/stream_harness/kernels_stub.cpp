@@ -40,10 +40,11 @@ size_t fp64_plane_bytes_per_wave(int max_hap_len) {
 
 size_t max_lds_per_block() { return 163840; }
 
-LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed) {
+LaunchShape forward_shape(bool fp64, int max_hap_len, int requested_waves_per_block, bool packed, int unroll) {
     LaunchShape sh;
     if (lds_columns(max_hap_len) == 0) return sh;
     sh.packed = packed && !fp64;
+    sh.unroll = unroll == 16 ? 16 : 8;
     const size_t per_wave = lds_bytes_per_wave(max_hap_len, fp64, sh.packed);
     int wpb = requested_waves_per_block < 1 ? 1 : requested_waves_per_block;
     while (wpb > 1 && per_wave * static_cast<size_t>(wpb) > max_lds_per_block()) --wpb;

@@ -7,7 +7,8 @@ rows in one stripe, lanes left idle past the second pair), reads ending at 61/62
 summing row at a stripe's last lanes and the next pair's first rows at lane 0), empty reads, 'N'
 haplotype bytes, and haplotypes holding the bytes the packed schedule reserves (0 and 1), which
 run aligned whatever the tuning says.  Packed results are identical from run to run, whatever
-the wave processed before (the side pass reads only the plane slots the previous stripe wrote).
+the wave processed before (the side pass reads only the plane slots the previous stripe wrote),
+and the packed kernel's two loop bodies (8 or 16 wave-steps) agree bit for bit.
 """
 import numpy as np
 import pytest
@@ -69,6 +70,33 @@ def test_stripe_schedules_match_oracle(stripes):
     assert inf.any() and np.array_equal(np.isinf(outs[0]), inf)
     assert np.max(np.abs(outs[0][~inf] - want[~inf])) <= 1e-4
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_packed_loop_bodies_agree_bit_for_bit():
+    """The packed kernel's 8- and 16-step loop bodies compute every cell with the same operations;
+    the longer body only adds steps past the haplotype's end.  With one pair per queue chunk (a small
+    batch) every pair's rows fall on the same lanes, so both bodies, and the per-class automatic
+    choice, give the same bits, within 1e-4 of the oracle.  The 1,000-base haplotypes fall in an LDS
+    class where the automatic choice is the 16-step body."""
+    rng = np.random.default_rng(17)
+    b = pack_regions([_region(rng, [1, 9, 61, 62, 63, 64, 65, 130]),
+                      _region(rng, [0, 70, 150], hap_n=True),
+                      _region(rng, [90, 200, 251], hap_len=1000, n_haps=4)])
+    want = oracle.logless_batch(b, n_threads=8)
+    outs = {}
+    with BatchEngine() as eng:
+        for unroll in (8, 16, 0):
+            Schedule(stripe_schedule=0, step_unroll=unroll).apply(eng)
+            dev = eng.upload(b)
+            o = np.empty(b.n_pairs)
+            dev.forward(mode=_native.RUN_PERSISTENT_EAGER, out=o)
+            dev.free()
+            outs[unroll] = o
+    inf = np.isinf(want)
+    assert inf.any() and np.array_equal(np.isinf(outs[8]), inf)
+    assert np.max(np.abs(outs[8][~inf] - want[~inf])) <= 1e-4
+    assert np.array_equal(outs[8].view(np.int64), outs[16].view(np.int64))
+    assert np.array_equal(outs[8].view(np.int64), outs[0].view(np.int64))
 
 
 def test_reserved_haplotype_bytes_run_aligned():
@@ -149,3 +177,19 @@ def test_packed_results_do_not_depend_on_the_waves_history():
             rescued[stripes] = counts[0]
     print(f"rescued: packed {rescued[0]}, aligned {rescued[1]}")
     assert abs(rescued[0] - rescued[1]) <= 0.05 * rescued[1] + 10
+
+
+def test_tuning_round_trip_and_bad_values():
+    """pairhmm_set_tuning stores what pairhmm_get_tuning returns and rejects values outside each
+    knob's range (the loop body takes 0, 8 or 16), leaving the previous tuning in force."""
+    import ctypes
+    with BatchEngine() as eng:
+        good = _native.Tuning(2, 150, 1, 0, 16)
+        assert eng.lib.pairhmm_set_tuning(eng.ctx, ctypes.byref(good)) == 0
+        for bad in (_native.Tuning(2, 150, 1, 0, 12), _native.Tuning(2, 150, 1, 2, 0),
+                    _native.Tuning(9, 150, 1, 0, 0), _native.Tuning(2, 150, 3, 0, 0)):
+            assert eng.lib.pairhmm_set_tuning(eng.ctx, ctypes.byref(bad)) == _native.PAIRHMM_ERR_BAD_ARG
+        got = _native.Tuning()
+        assert eng.lib.pairhmm_get_tuning(eng.ctx, ctypes.byref(got)) == 0
+        assert (got.waves_per_block, got.grid_percent, got.queue_order, got.stripe_schedule, got.step_unroll) == \
+            (2, 150, 1, 0, 16)
