@@ -254,9 +254,13 @@ struct SwBatch {
     size_t scratch_per_wave, bt_bytes_per_wave;
     uint32_t *elems;                        // CIGAR elements (len << 2 | op), reference list order
     const int64_t *elem_off;                // per pair; capacity ref_len + alt_len + 4
+    uint32_t *elems_short;                  // the first kSwShortElems elements of pair p at p * kSwShortElems
     int32_t *n_elems;
     int32_t *offset;                        // alignment offset
 };
+// Elements per pair in the compact copy the host reads back; a pair with more (rare: every element
+// but a clip consumes an indel or a run of matches) is read from the full list.
+constexpr int kSwShortElems = 16;
 size_t sw_lds_bytes(int max_ref_len);
 size_t sw_bt_bytes_per_wave(int max_ref_len, int max_alt_len);
 size_t sw_scratch_bytes_per_wave(int max_ref_len, int max_alt_len);

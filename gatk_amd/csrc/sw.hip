@@ -205,9 +205,14 @@ __global__ __launch_bounds__(kWave) void sw_kernel(SwBatch b, uint32_t n_pairs) 
             }
         }
         uint32_t *elems = b.elems + b.elem_off[p];
+        uint32_t *elems_short = b.elems_short + static_cast<size_t>(p) * kSwShortElems;
         int n = 0;
         auto put = [&](uint32_t op, int32_t len) {
-            if (lane == 0) elems[n] = (static_cast<uint32_t>(len) << 2) | op;
+            const uint32_t e = (static_cast<uint32_t>(len) << 2) | op;
+            if (lane == 0) {
+                elems[n] = e;
+                if (n < kSwShortElems) elems_short[n] = e;
+            }
             ++n;
         };
         if (segment > 0 && b.strategy == 0) {

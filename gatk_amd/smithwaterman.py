@@ -95,12 +95,14 @@ class HipSmithWatermanBinding:
         n = len(refs)
         if n != len(alts):
             raise ValueError("refs and alts differ in length")
-        rl = np.array([len(x) for x in refs], np.int32)
-        al = np.array([len(x) for x in alts], np.int32)
+        rl = np.fromiter(map(len, refs), np.int32, n)
+        al = np.fromiter(map(len, alts), np.int32, n)
         ro = np.concatenate([[0], np.cumsum(rl, dtype=np.int64)[:-1]]).astype(np.int64)
         ao = np.concatenate([[0], np.cumsum(al, dtype=np.int64)[:-1]]).astype(np.int64)
-        rb = np.frombuffer(b"".join(bytes(x) for x in refs), np.uint8).copy()
-        ab = np.frombuffer(b"".join(bytes(x) for x in alts), np.uint8).copy()
+        # bytes.join takes any buffer (bytes, bytearray, uint8 arrays) without a per-item copy; the
+        # library only reads the joined bases, so the read-only views are passed as they are.
+        rb = np.frombuffer(b"".join(refs), np.uint8) if n else np.zeros(0, np.uint8)
+        ab = np.frombuffer(b"".join(alts), np.uint8) if n else np.zeros(0, np.uint8)
         return n, rl, al, ro, ao, rb, ab
 
     def align_batch(self, refs: Sequence[bytes], alts: Sequence[bytes], parameters: SWParameters,

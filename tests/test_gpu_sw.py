@@ -87,6 +87,36 @@ def test_long_sequences(binding):
         assert g == oracle.sw_align(r, a, _p(NEW_SW_PARAMETERS), "SOFTCLIP")
 
 
+def test_many_element_cigars(binding):
+    """Reads with many indels, interleaved with plain reads: CIGARs of 1 to about 80 elements, so
+    the batch mixes pairs read back from the compact first-16-element copy with pairs whose full
+    element lists come back one by one."""
+    rng = np.random.default_rng(23)
+    refs, alts, n_el = [], [], []
+    for k in range(120):
+        H = int(rng.integers(300, 900))
+        hap = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, H)])
+        read = bytearray(hap[20:H - 20])
+        sites = np.arange(10, len(read) - 10, 12)
+        n_indels = 0 if k % 3 == 0 else min(int(rng.integers(4, 40)), len(sites))
+        for pos in sorted(rng.choice(sites, n_indels, replace=False), reverse=True):
+            if rng.random() < 0.5:
+                del read[pos:pos + int(rng.integers(1, 4))]
+            else:
+                read[pos:pos] = bytes(np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, int(rng.integers(1, 4)))])
+        refs.append(hap)
+        alts.append(bytes(read))
+    for strategy in SWOverhangStrategy:
+        print(f"[sw] many-element CIGARs strategy={strategy.name}", flush=True)
+        got = binding.align_batch(refs, alts, ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS, strategy)
+        assert got == binding.align_batch_buffers(refs, alts, ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS, strategy)
+        for r, a, g in zip(refs, alts, got):
+            assert g == oracle.sw_align(r, a, _p(ALIGNMENT_TO_BEST_HAPLOTYPE_SW_PARAMETERS), strategy.name)
+            n_el.append(sum(ch.isalpha() for ch in g[0]))
+    n_el = np.array(n_el)
+    assert (n_el <= 16).any() and (n_el > 16).sum() >= 40 and n_el.max() >= 40, np.bincount(n_el)
+
+
 def test_empty_sequence_is_an_error(binding):
     with pytest.raises(ValueError):
         binding.align(b"", b"ACGT", ORIGINAL_DEFAULT, SWOverhangStrategy.SOFTCLIP)
