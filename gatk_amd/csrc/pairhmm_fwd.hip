@@ -470,8 +470,10 @@ __global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTable
             process_pair<T, kCols>(b, tab, rs, L, lane, ws.items ? ws.items[it] : it);
         }
     } else {
+        // The wave's index is wave-uniform; readfirstlane tells the compiler so, which keeps the
+        // pair, its lengths and every loop over them scalar (uniform branches, no exec masking).
         const uint32_t stride = gridDim.x * waves_per_block;
-        for (uint32_t it = gwave; it < n_items; it += stride)
+        for (uint32_t it = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(gwave)); it < n_items; it += stride)
             process_pair<T, kCols>(b, tab, rs, L, lane, ws.items ? ws.items[it] : it);
     }
 }

@@ -183,6 +183,26 @@ def test_edge_cases_match_oracle():
         assert np.all(out <= 0)
 
 
+def test_fp64_kernel_every_read_length_to_130(engine64):
+    """The FP64 kernel against the oracle at 1e-9 for every read length
+    1-130 (rows ending on every lane of a first and second stripe) on haplotypes of 3, 10, 40 and 70
+    bases, whose column counts put a read's last rows in different 8-step blocks."""
+    rng = np.random.default_rng(5)
+    regions = []
+    for R in range(1, 131):
+        haps = [rng.choice(list(b"ACGT"), H).astype(np.uint8) for H in (3, 10, 40, 70)]
+        bases = rng.choice(list(b"ACGT"), R).astype(np.uint8)
+        regions.append((haps, [(bases, rng.integers(20, 41, R).astype(np.uint8), np.full(R, 45, np.uint8),
+                                np.full(R, 45, np.uint8), np.full(R, 10, np.uint8))]))
+    b = pack_regions(regions)
+    want = oracle.logless_batch(b, n_threads=8)
+    dev = engine64.upload(b)
+    out = np.empty(b.n_pairs)
+    dev.forward(mode=_native.RUN_FP64, out=out)
+    dev.free()
+    assert np.max(np.abs(out - want)) <= FP64_TOL
+
+
 def _repeat_pairs(read_unit, ref_unit, copies):
     """Reads and haplotypes of repeated units with the qualities of PairHMMUnitTest's big-read case
     (base 30, insertion 40, deletion 40, gap continuation 10)."""
