@@ -45,7 +45,7 @@ template <typename T>
 struct WaveLds {
     uint8_t *hap;      // [cols]: haplotype byte of column c (1-based) at kLdsPad + c - 1
     T *bm, *bi;        // [cols]: boundary planes, column c at kLdsPad + c
-    T *ring;           // FP64: LDS ring of the rows' M, S, I (kRing64 slots each)
+    T *ring;           // FP64: LDS ring of the rows' (B, A) sums for the next row (kRing64 slots)
 };
 
 template <int kCols>
@@ -62,11 +62,11 @@ __device__ __forceinline__ int plane_span(int steps) {
 // Two boundary planes per wave: lane 63's row (FP32: M and I in LDS, turned into lane 0's sums A
 // and B by the side pass; FP64: A and B formed by lane 63 itself, in global memory).
 constexpr int kPlanes = 2;
-// FP64 step loop: every lane stores its M, S and I at its own column into an LDS ring of 128 slots
-// (plus 8 for a block that runs past slot 127), from which each 8-step block forms lane 0's sums
-// of lane 63's finished columns.
+// FP64 step loop: every lane stores the sums it forms for the row below, (B(c), A(c+1)), at its
+// own column into an LDS ring of 128 slots (plus 8 for a block that runs past slot 127), from which
+// each 8-step block copies lane 63's finished columns out as lane 0's sums of the next stripe.
 constexpr int kRing64 = 136;
-constexpr size_t kRing64Bytes = 3 * kRing64 * sizeof(double);
+constexpr size_t kRing64Bytes = kRing64 * 2 * sizeof(double);
 
 template <typename T, int kCols>
 constexpr size_t lds_per_wave() {
@@ -261,20 +261,23 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
     return wave_sum(acc);
 }
 
-// FP64 pass.  Lane 0's sums of the row above for the next stripe,
-//   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II'
-// over lane 63's row, with the next stripe's first-row transitions (MM', GM', MI', II'), which are
-// wave-uniform.  After the last stripe they are the pass-through row's (0, 0, 1, 1), so the B plane
-// ends up holding M(R,c) + I(R,c).  Lane 0 takes A and B as the old values of the DPP moves
-// (wave_shr:1 without bound_ctrl leaves lane 0 unwritten) with its own MM = MI = 1, so the step has
-// no lane-0 select.
+// FP64 pass.  Each lane forms the sums the row below it needs from its own new M, S and I, with the
+// transitions of the row below (the next lane's row; for lane 63 the next stripe's first row):
+//   A(c+1) = M(c) MM' + S(c) GM'   and   B(c) = M(c) MI' + I(c) II',
+// and the row below takes them one lane over: B at the next step (same column), A two steps later
+// (the next column).  So two doubles cross lanes per step (four 32-bit DPP moves) where the
+// products over the row above would move four (M and S two steps back, M and I one step back), and
+// a lane needs only the next row's MM, GM, MI, II and its own MD, DD and priors.  Lane 63's sums are
+// lane 0's A and B of the next stripe; after the last stripe the next row is the pass-through row
+// (MM = GM = 0, MI = II = 1), so the B plane ends up holding M(R,c) + I(R,c).  Lane 0 takes A and B
+// as the old values of the DPP moves (wave_shr:1 without bound_ctrl leaves lane 0 unwritten).
 //
-// Every lane stores its M, S and I at its own column c = t - lane into an LDS ring at step t (no
+// Every lane stores its (B(c), A(c+1)) at its own column c = t - lane into an LDS ring at step t (no
 // one-lane exec mask in the step); lane 63 writes column c last, at step c + 63.  After each block
-// of 8 steps the whole wave turns lane 63's 8 columns finished in it into their A and B and stores
-// them, one 16-byte store per column (B(c), A(c+1)), with every lane storing (lanes 8-63 repeat
-// lanes 0-7).  A lane's ring slots for block t0 start at (t0 - lane) mod 128, so a block never wraps
-// and two live columns (less than 128 apart) never share a slot.
+// of 8 steps the whole wave copies lane 63's 8 columns finished in it out to the plane, one 16-byte
+// store per column, with every lane storing (lanes 8-63 repeat lanes 0-7).  A lane's ring slots for
+// block t0 start at (t0 - lane) mod 128, so a block never wraps and two live columns (less than 128
+// apart) never share a slot.
 //
 // A and B live in the wave's slice of global memory rather than LDS (an FP64 plane is 8 bytes a
 // column; in LDS the long-haplotype classes held a CU to 5-7 waves); the slice stays in L2.  They
@@ -285,6 +288,27 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
 // is reloaded with the next block's first four columns, after step 7 the second half.  A wait for
 // the wave's stores at each stripe start orders the previous stripe's stores before these loads;
 // all accesses come from one wave on one CU, so its L1 holds no stale copy.
+// a, b <- the previous lane's sa, sb (lane 0 keeps a, b), as four 32-bit DPP moves after s_nop 4.
+__device__ __forceinline__ void shr1_pair_f64(double &a, double sa, double &b, double sb) {
+    uint32_t a0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, a)),
+             a1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, a) >> 32),
+             b0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, b)),
+             b1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, b) >> 32);
+    const uint32_t s0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sa)),
+                   s1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sa) >> 32),
+                   t0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sb)),
+                   t1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sb) >> 32);
+    asm("s_nop 4\n\t"
+        "v_mov_b32_dpp %0, %4 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %1, %5 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %2, %6 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %3, %7 wave_shr:1 row_mask:0xf bank_mask:0xf"
+        : "+&v"(a0), "+&v"(a1), "+&v"(b0), "+&v"(b1)
+        : "v"(s0), "v"(s1), "v"(t0), "v"(t1));
+    a = __builtin_bit_cast(double, (static_cast<uint64_t>(a1) << 32) | a0);
+    b = __builtin_bit_cast(double, (static_cast<uint64_t>(b1) << 32) | b0);
+}
+
 template <int kCols, bool kHapHasN>
 __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                    const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
@@ -296,31 +320,24 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
     using lds_u8 = const volatile __attribute__((address_space(3))) uint8_t;
     lds_u8 *hp = (lds_u8 *)(L.hap + kLdsPad - 1 - lane);  // address-space cast: C-style only
     using lds_f64 = __attribute__((address_space(3))) double;
-    lds_f64 *const rM = (lds_f64 *)L.ring, *const rS = rM + kRing64, *const rI = rS + kRing64;
+    lds_f64 *const ring = (lds_f64 *)L.ring;  // slot k: B at 2 k, A at 2 k + 1
     constexpr int kLag = kWave - 1;
-    int lane_r = lane, zoff;
-    asm volatile("" : "+v"(lane_r));
+    int zoff;
     asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
-    const bool is_first = lane_r == 0;
     const double *r0 = P + 2 * kLdsPad + zoff;
     // Copy-out: lane handles column offset v of the block's finished columns.
     const int v = lane & (kUnroll - 1);
     for (int s = 0; s < n_stripes; ++s) {
-        RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
-        const RowTrans<double> nx = row_trans<double>((s + 1) * kWave, R, rb, rq, ri, rd, rg, tab);
-        if (is_first) {
-            w.MM = 1.0;
-            w.MI = 1.0;
-        }
-        double m1 = 0, i1 = 0, d1 = 0, s1 = 0;
-        double m2 = 0, s2 = 0;
+        const RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
+        const RowTrans<double> wn = row_trans<double>(s * kWave + lane + 1, R, rb, rq, ri, rd, rg, tab);
+        double m1 = 0, d1 = 0;
+        double an1 = 0, an2 = 0, bn1 = 0;  // sums for the row below, one and two steps back
         // The previous stripe's (or the initial) stores complete before this stripe reads them.
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         double ab[2 * kUnroll];
         __builtin_memcpy(ab, __builtin_assume_aligned(r0, 16), sizeof(ab));
         for (int t0 = 0; t0 < steps; t0 += kUnroll) {
-            const int rb0 = (t0 - lane) & 127;
-            lds_f64 *wM = rM + rb0, *wS = rS + rb0, *wI = rI + rb0;
+            lds_f64 *wr = ring + 2 * ((t0 - lane) & 127);
             uint32_t hbs[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) hbs[u] = hp[t0 + u];
@@ -330,20 +347,20 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 const uint32_t hb = hbs[u];
                 const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
                 const double prior = match ? w.pm : w.px;
-                const double a = fma(from_prev_lane(s2), w.GM, wave_shr1(ab[2 * u], m2) * w.MM);
-                const double i = fma(from_prev_lane(i1), w.II, wave_shr1(ab[2 * u + 1], m1) * w.MI);
+                double a = ab[2 * u], i = ab[2 * u + 1];
+                shr1_pair_f64(a, an2, i, bn1);
                 const double m = a * prior;
                 const double d = fma(d1, w.DD, m1 * w.MD);
                 const double sx = i + d;
-                wM[u] = m;
-                wS[u] = sx;
-                wI[u] = i;
-                m2 = m1;
-                s2 = s1;
+                const double an = fma(sx, wn.GM, m * wn.MM);
+                const double bn = fma(i, wn.II, m * wn.MI);
+                wr[2 * u] = bn;
+                wr[2 * u + 1] = an;
+                an2 = an1;
+                an1 = an;
+                bn1 = bn;
                 m1 = m;
-                i1 = i;
                 d1 = d;
-                s1 = sx;
                 // The next block's halves, loaded unconditionally (the slice has slots past the last
                 // block): with a fixed count of loads in the loop, the wait for a half is a counted
                 // one instead of a full drain at the block start.
@@ -355,14 +372,11 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
             }
             __builtin_amdgcn_wave_barrier();
             // Lane 63 finished columns t0 - 63 .. t0 - 56 in this block, at ring slots
-            // ((t0 - 63) mod 128) + v.
+            // ((t0 - 63) mod 128) + v: (B(c), A(c + 1)) as they go to the plane.
             {
-                const int slot = ((t0 - kLag) & 127) + v;
-                const double mc = rM[slot], sc = rS[slot], ic = rI[slot];
-                double2 ba;
-                ba.x = fma(ic, nx.II, mc * nx.MI);  // B(c)
-                ba.y = fma(sc, nx.GM, mc * nx.MM);  // A(c + 1)
-                __builtin_memcpy(P + 2 * (kLdsPad + t0 - kLag + v) + 1, &ba, sizeof(ba));
+                const lds_f64 *rs = ring + 2 * (((t0 - kLag) & 127) + v);
+                const double ba[2] = {rs[0], rs[1]};
+                __builtin_memcpy(P + 2 * (kLdsPad + t0 - kLag + v) + 1, ba, sizeof(ba));
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -449,7 +463,8 @@ __device__ void process_pair(const DeviceBatch &b, const DeviceTables &tab, cons
 }
 
 template <typename T, int kCols>
-__global__ __launch_bounds__(512) void forward_kernel(DeviceBatch b, DeviceTables tab, WorkSource ws, RescueSink rs) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == sizeof(double) ? 4 : 1)))
+void forward_kernel(DeviceBatch b, DeviceTables tab, WorkSource ws, RescueSink rs) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;

@@ -203,6 +203,31 @@ def test_fp64_kernel_every_read_length_to_130(engine64):
     assert np.max(np.abs(out - want)) <= FP64_TOL
 
 
+def test_fp64_kernel_random_shapes(engine64):
+    """The FP64 kernel against the oracle at 1e-9 on 240 random pairs: reads of 1-600 bases, haplotypes
+    of 1-1,500 (several column classes and stripe counts), base qualities 2-40."""
+    rng = np.random.default_rng(23)
+    regions = []
+    for _ in range(60):
+        haps = [rng.choice(list(b"ACGT"), int(rng.integers(1, 1501))).astype(np.uint8) for _ in range(2)]
+        reads = []
+        for _ in range(2):
+            R = int(rng.integers(1, 601))
+            reads.append((rng.choice(list(b"ACGT"), R).astype(np.uint8), rng.integers(2, 41, R).astype(np.uint8),
+                          rng.integers(20, 60, R).astype(np.uint8), rng.integers(20, 60, R).astype(np.uint8),
+                          rng.integers(5, 20, R).astype(np.uint8)))
+        regions.append((haps, reads))
+    b = pack_regions(regions)
+    want = oracle.logless_batch(b, n_threads=8)
+    dev = engine64.upload(b)
+    out = np.empty(b.n_pairs)
+    dev.forward(mode=_native.RUN_FP64, out=out)
+    dev.free()
+    fin = np.isfinite(want)
+    assert np.array_equal(np.isfinite(out), fin)
+    assert np.max(np.abs(out[fin] - want[fin])) <= FP64_TOL
+
+
 def _repeat_pairs(read_unit, ref_unit, copies):
     """Reads and haplotypes of repeated units with the qualities of PairHMMUnitTest's big-read case
     (base 30, insertion 40, deletion 40, gap continuation 10)."""
