@@ -288,27 +288,6 @@ __device__ double forward_pair_f32(const WaveLds<T> &L, int lane, int H, int R, 
 // is reloaded with the next block's first four columns, after step 7 the second half.  A wait for
 // the wave's stores at each stripe start orders the previous stripe's stores before these loads;
 // all accesses come from one wave on one CU, so its L1 holds no stale copy.
-// a, b <- the previous lane's sa, sb (lane 0 keeps a, b), as four 32-bit DPP moves after s_nop 4.
-__device__ __forceinline__ void shr1_pair_f64(double &a, double sa, double &b, double sb) {
-    uint32_t a0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, a)),
-             a1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, a) >> 32),
-             b0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, b)),
-             b1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, b) >> 32);
-    const uint32_t s0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sa)),
-                   s1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sa) >> 32),
-                   t0 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sb)),
-                   t1 = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sb) >> 32);
-    asm("s_nop 4\n\t"
-        "v_mov_b32_dpp %0, %4 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
-        "v_mov_b32_dpp %1, %5 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
-        "v_mov_b32_dpp %2, %6 wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"
-        "v_mov_b32_dpp %3, %7 wave_shr:1 row_mask:0xf bank_mask:0xf"
-        : "+&v"(a0), "+&v"(a1), "+&v"(b0), "+&v"(b1)
-        : "v"(s0), "v"(s1), "v"(t0), "v"(t1));
-    a = __builtin_bit_cast(double, (static_cast<uint64_t>(a1) << 32) | a0);
-    b = __builtin_bit_cast(double, (static_cast<uint64_t>(b1) << 32) | b0);
-}
-
 template <int kCols, bool kHapHasN>
 __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, int R, const uint8_t *__restrict__ rb,
                                    const uint8_t *__restrict__ rq, const uint8_t *__restrict__ ri,
@@ -348,7 +327,8 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 const bool match = (hb == w.base) || (kHapHasN && hb == 'N');
                 const double prior = match ? w.pm : w.px;
                 double a = ab[2 * u], i = ab[2 * u + 1];
-                shr1_pair_f64(a, an2, i, bn1);
+                a = wave_shr1(a, an2);  // lane 0 keeps its plane values
+                i = wave_shr1(i, bn1);
                 const double m = a * prior;
                 const double d = fma(d1, w.DD, m1 * w.MD);
                 const double sx = i + d;
