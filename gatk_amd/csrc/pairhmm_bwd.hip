@@ -315,8 +315,10 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
         // column's stores come after lane 63 has read it, and lane 0's, at step c, come last.  Lane
         // 63's X store alone would land before its own read of that column one step later; it goes
         // one column right, onto the column it has just read.
-        const T *bxr = L.bx + kLdsPad - (kLag - 1), *bgr = L.bg + kLdsPad - kLag;
-        using lds_t = __attribute__((address_space(3))) T;
+        // Volatile: single-dword LDS accesses at immediate offsets (paired ds_read2 / ds_write2 forms
+        // cannot reach the planes' offsets and cost an address add each).
+        using lds_t = volatile __attribute__((address_space(3))) T;
+        const lds_t *bxr = (const lds_t *)(L.bx + kLdsPad - (kLag - 1)), *bgr = (const lds_t *)(L.bg + kLdsPad - kLag);
         lds_t *bxw = (lds_t *)(L.bx + kLdsPad) - lane_x + (lane_x == kLag ? 1 : 0);
         lds_t *bgw = (lds_t *)(L.bg + kLdsPad) - lane_x;
         T X1 = 0, X2 = 0, gI1 = 0, gD1 = 0;
@@ -340,6 +342,16 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             uint32_t hbs[kTile];
 #pragma unroll
             for (int u = 0; u < kTile; ++u) hbs[u] = hp[t0 + u];
+            // Lane 63's row-below values of the tile, read at its start: column c of the X plane is
+            // stored by lane k at step c + k (lane 63 at step c + 62, after its read there), and of
+            // the gI plane at step c + k, so no store of the tile reaches a column before the step
+            // that reads it (steps run t0 + kTile - 1 down to t0).
+            T xv[kTile], gv[kTile];
+#pragma unroll
+            for (int u = 0; u < kTile; ++u) {
+                xv[u] = bxr[t0 + u];
+                gv[u] = bgr[t0 + u];
+            }
             __builtin_amdgcn_wave_barrier();
             T stM[kTile], stI[kTile], stD[kTile], stS[kTile], stP[kTile], stAW[kTile];
 #pragma unroll
@@ -353,8 +365,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 #pragma unroll
             for (int u = kTile - 1; u >= 0; --u) {
                 const int t = t0 + u;
-                const T Xn2 = wave_shl1(bxr[t], X2);   // X(r+1, c+1)
-                const T gIn1 = wave_shl1(bgr[t], gI1); // gI(r+1, c)
+                const T Xn2 = wave_shl1(xv[u], X2);   // X(r+1, c+1)
+                const T gIn1 = wave_shl1(gv[u], gI1); // gI(r+1, c)
                 const T gS = GMn * Xn2;
                 const T gD = fma(w.DD, gD1, gS);
                 const T gI = fma(IIn, gIn1, gS);
