@@ -194,9 +194,12 @@ int dragstr_max_read_length();
 // its PartiallyDeterminedHaplotype byte): bits 0-7 the base; bits 8-15 the bases the column also
 // accepts (the SNP alternatives A/C/G/T at the PD byte's own bit positions, or every class bit for
 // an 'N' base); kPdDelEnd when the base ends a deletion span; kPdInsideRow1 / kPdInsideLater when
-// row 1 / rows >= 2 process the column in state INSIDE_DEL.
+// row 1 / rows >= 2 process the column in state INSIDE_DEL; kPdSpecial on every column that carries
+// kPdDelEnd or an inside bit and on the column before any column with an inside bit (a deletion
+// start): the only columns where a row's branch values are read or must be set.
 constexpr uint32_t kPdAltA = 8, kPdAltC = 16, kPdAltG = 32, kPdAltT = 64;
 constexpr uint32_t kPdDelEnd = 1u << 24, kPdInsideRow1 = 1u << 25, kPdInsideLater = 1u << 26;
+constexpr uint32_t kPdSpecial = 1u << 27;
 
 // Every read against every haplotype; out[r * n_haps + h] (read-major).
 struct PdBatch {
@@ -212,7 +215,8 @@ struct PdBatch {
     const uint32_t *order;     // launch order of the pairs (longest first), or null
 };
 
-// LDS of one wave (one workgroup) for haplotypes up to max_hap_len: one 32-byte record per column.
+// LDS of one wave (one workgroup) for haplotypes up to max_hap_len: a 16-byte record and a 4-byte
+// column code per column.
 size_t pd_lds_bytes(int max_hap_len);
 hipError_t launch_pd_forward(const PdBatch &batch, const DeviceTables &tables, uint32_t n_pairs, int max_hap_len,
                              int grid_blocks, hipStream_t stream);

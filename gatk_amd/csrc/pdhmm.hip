@@ -21,12 +21,18 @@
 // same for all rows >= 2.  The host folds this into each column's code (pd_column_codes).
 //
 // Mapping: one pair per wave, stripes of 64 rows, lane k owns row 64 s + k + 1 and at step t
-// computes column c = t - k + 1.  Lane k-1's E values of the last step arrive by DPP wave_shr:1 (the
-// cell above); the ones of the step before are the diagonal.  Lane 0 reads the row above from an
-// LDS record per column that lane 63 of the previous stripe wrote (E_M, E_I, E_D and the column's
-// code; stripe 0 starts from row 0: M = I = 0, D = 2^1020 / H).  Column codes ride down the wave
-// with the values, so only lane 0 reads LDS.  The row-R lane sums M + I over columns 1..H in
-// column order, as the reference does (:132-139).
+// computes column c = t - k + 1.  Each lane forms, from its new E values and the transitions of the
+// row below (the next lane's row; for lane 63 the next stripe's first row), the two sums that row
+// needs:
+//   A(r+1, c+1) = E_M(r,c) MM' + (E_I(r,c) + E_D(r,c)) GM'    (so M(r+1,c+1) = prior' A)
+//   B(r+1, c)   = E_M(r,c) MI' + E_I(r,c) II'                 (so I(r+1,c) = B)
+// and the row below takes A two steps later and B one step later by DPP wave_shr:1: two doubles
+// cross lanes per step instead of the three E values.  Lane 0 reads its A and B from an LDS record
+// per column that lane 63 of the previous stripe wrote (stripe 0 starts from row 0: M = I = 0,
+// D = 2^1020 / H, so A(1,c) = D GM_1 and B(1,c) = 0); lane 0 also reads the column code, which rides
+// down the wave with the values.  Records are 16 bytes (A, B) plus a 4-byte code array, so a wave
+// holds 20 bytes of LDS per column.  The row-R lane sums M + I over columns 1..H in column order, as
+// the reference does (:132-139), in the last stripe only.
 #include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
 
@@ -35,14 +41,16 @@ namespace {
 
 using namespace dev;
 
-// One column of a wave's LDS record array: the E values of the row above and the column code.
+// One column of a wave's LDS records: A and B of the stripe's first row (the row below lane 63 of
+// the stripe before).  The column codes are a separate uint32 array after the records.
 struct alignas(16) PdRecord {
-    double em, ei, ed;
-    uint32_t code, pad;
+    double a, b;
 };
-static_assert(sizeof(PdRecord) == 32, "PdRecord is two 16-byte LDS words");
+static_assert(sizeof(PdRecord) == 16, "PdRecord is one 16-byte LDS word");
 
-constexpr int kPdPadColumns = 66;  // records past H that lane 0 reads ahead (c <= H + 64)
+constexpr int kPdPadColumns = 68;  // records past H that lane 0 reads (c <= H + 66 with 4-step blocks)
+constexpr int kPdLowPad = 64;      // records below column 0, where lane 63's stores of columns < 0 land
+constexpr int kPdU = 4;            // wave-steps per unrolled block
 
 __device__ __forceinline__ uint32_t shr1_keep(uint32_t old, uint32_t src) {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(src),
@@ -62,10 +70,120 @@ __device__ __forceinline__ uint32_t read_base_bit(uint32_t x) {
     }
 }
 
+// Row parameters of read position p (row p + 1); rows past the read are zero.
+struct PdRow {
+    double MM, GM, MI, II, MD, DD, pm, px;
+    uint32_t x, xbit;
+};
+
+__device__ __forceinline__ PdRow pd_row(const PdBatch &b, const DeviceTables &tab, int64_t ro, int R, int p) {
+    PdRow w{0, 0, 0, 0, 0, 0, 0, 0, 0u, 0u};
+    if (p < 0 || p >= R) return w;
+    w.x = b.rd_bases[ro + p];
+    const int q = b.rd_quals[ro + p], qi = b.rd_ins[ro + p], qd = b.rd_del[ro + p], qc = b.rd_gcp[ro + p];
+    const int mn = qi < qd ? qi : qd, mx = qi < qd ? qd : qi;
+    w.MM = tab.mm_prob[((mx * (mx + 1)) >> 1) + mn];
+    w.MI = tab.qual_err[qi];
+    w.MD = tab.qual_err[qd];
+    w.II = tab.qual_err[qc];
+    w.DD = w.II;
+    w.GM = 1.0 - w.II;
+    const double eq = tab.qual_err[q];
+    w.pm = 1.0 - eq;
+    w.px = b.tristate ? eq / 3.0 : eq;
+    if (w.x == 'N') w.px = w.pm;
+    w.xbit = read_base_bit(w.x);
+    return w;
+}
+
+// Lane state of a stripe: own E and branch values at column c-1, the row-below sums of the last
+// two steps, the last-row sum.
+struct PdLane {
+    double EM = 0, EI = 0, ED = 0, bM = 0, bI = 0, bD = 0;
+    double an1 = 0, an2 = 0, bn1 = 0, acc = 0;
+};
+
+// One step of a lane at column c.  kBranch: the full branch update; without it E = X and the branch
+// values are left alone, which is exact on a block where no lane's column is special: a row's branch
+// values are only read at its special columns, and a deletion start (special) sets them from E.
+template <bool kLast, bool kBranch>
+__device__ __forceinline__ void pd_step(PdLane &st, const PdRecord &cur, uint32_t code, int c, int H, int R, int row,
+                                        const PdRow &w, const PdRow &wn, uint32_t inside_bit, bool writer,
+                                        PdRecord *wr) {
+    const double a = wave_shr1(cur.a, st.an2);  // lane 0 keeps its record's values
+    const double I = wave_shr1(cur.b, st.bn1);
+    const bool match = ((code & 0xFFu) == w.x) || (code & w.xbit) != 0u;
+    const double M = (match ? w.pm : w.px) * a;
+    const double D = fma(st.ED, w.DD, st.EM * w.MD);
+    if constexpr (kBranch) {
+        const bool inside = (code & inside_bit) != 0u;
+        st.bM = inside ? st.bM : st.EM;
+        st.bI = inside ? st.bI : st.EI;
+        st.bD = inside ? st.bD : st.ED;
+        const double f = (code & kPdDelEnd) ? 1.0 : 0.0;  // max(0, X) == X for the cells here
+        st.EM = fmax(st.bM * f, M);
+        st.EI = fmax(st.bI * f, I);
+        st.ED = fmax(st.bD * f, D);
+    } else {
+        st.EM = M;
+        st.EI = I;
+        st.ED = D;
+    }
+    const double an = fma(st.EI + st.ED, wn.GM, st.EM * wn.MM);
+    const double bn = fma(st.EI, wn.II, st.EM * wn.MI);
+    st.an2 = st.an1;
+    st.an1 = an;
+    st.bn1 = bn;
+    if (writer) {  // (B(c), A(c + 1)) of the row below; c >= -62 (low pad)
+        wr->b = bn;
+        wr[1].a = an;
+    }
+    if (kLast && row == R && c >= 1 && c <= H) st.acc += M + I;
+}
+
+// Steps of one stripe, in blocks of kPdU; kLast adds the row-R lane's sum of M + I.  Lane 0's
+// records of a block are read at its start (lane 63 writes record j at step j + 61 or later); every
+// lane reads its own columns' codes (cds[c], zero for c < 1 and c > H).  A block runs the branch
+// update only if some lane's column in it is special.
+template <bool kLast>
+__device__ __forceinline__ double pd_stripe(PdRecord *rec, const uint32_t *cds, int lane, int H, int R, int row,
+                                            const PdRow &w, const PdRow &wn, uint32_t inside_bit) {
+    PdLane st;
+    const bool writer = lane == kWave - 1;
+    const int steps = round_up(H + kWave - 1, kPdU);
+    const uint32_t *cp = cds + 1 - lane;  // column t - lane + 1 at step t
+    for (int t0 = 0; t0 < steps; t0 += kPdU) {
+        PdRecord cur[kPdU];
+        uint32_t cc[kPdU];
+        uint32_t any = 0;
+#pragma unroll
+        for (int u = 0; u < kPdU; ++u) {
+            cur[u] = rec[t0 + u + 1];
+            cc[u] = cp[t0 + u];
+            any |= cc[u];
+        }
+        PdRecord *wr = rec + (t0 - (kWave - 2));  // lane 63's column at step t0 is t0 - 62
+        const int c0 = t0 - lane + 1;
+        if (__builtin_amdgcn_ballot_w64((any & kPdSpecial) != 0u) != 0) {
+#pragma unroll
+            for (int u = 0; u < kPdU; ++u)
+                pd_step<kLast, true>(st, cur[u], cc[u], c0 + u, H, R, row, w, wn, inside_bit, writer, wr + u);
+        } else {
+#pragma unroll
+            for (int u = 0; u < kPdU; ++u)
+                pd_step<kLast, false>(st, cur[u], cc[u], c0 + u, H, R, row, w, wn, inside_bit, writer, wr + u);
+        }
+    }
+    return st.acc;
+}
+
 __global__ __launch_bounds__(kWave) void pd_forward_kernel(PdBatch b, DeviceTables tab, uint32_t n_pairs,
                                                             int rec_columns) {
-    extern __shared__ PdRecord rec[];
+    extern __shared__ PdRecord lds_rec[];
+    PdRecord *const rec = lds_rec + kPdLowPad;
+    uint32_t *const cds = reinterpret_cast<uint32_t *>(rec + rec_columns) + kPdLowPad;
     const int lane = static_cast<int>(threadIdx.x);
+    for (int c = lane - kPdLowPad; c < 0; c += kWave) cds[c] = 0u;  // columns below 1 carry no code
     const double log10_init = log10(ldexp(1.0, Scale<double>::kExp));
     for (uint32_t k = blockIdx.x; k < n_pairs; k += gridDim.x) {
         const uint32_t pair = b.order ? b.order[k] : k;
@@ -77,90 +195,33 @@ __global__ __launch_bounds__(kWave) void pd_forward_kernel(PdBatch b, DeviceTabl
             continue;
         }
         const uint32_t *codes = b.hap_code + b.hap_off[h];
+        const int64_t ro = b.rd_off[r];
         const double d0 = ldexp(1.0, Scale<double>::kExp) / static_cast<double>(H);
+        // Row 1's sums over row 0 (M = I = 0, D = d0 on columns 0..H): A(c) = d0 GM_1 for 1 <= c <= H + 1.
+        const double gm1 = pd_row(b, tab, ro, R, 0).GM;
         __builtin_amdgcn_wave_barrier();
         for (int c = lane; c < H + kPdPadColumns && c < rec_columns; c += kWave) {
-            PdRecord v;
-            v.em = 0.0;
-            v.ei = 0.0;
-            v.ed = c <= H ? d0 : 0.0;
-            v.code = (c >= 1 && c <= H) ? codes[c - 1] : 0u;
-            v.pad = 0u;
-            rec[c] = v;
+            // Two 8-byte stores (volatile, so they are not merged into one 16-byte store).
+            volatile double *v = &rec[c].a;
+            v[0] = (c >= 1 && c <= H + 1) ? d0 * gm1 : 0.0;
+            v[1] = 0.0;
+            cds[c] = (c >= 1 && c <= H) ? codes[c - 1] : 0u;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        const int64_t ro = b.rd_off[r];
         const int n_stripes = (R + kWave - 1) / kWave;
         double acc = 0.0;
         for (int s = 0; s < n_stripes; ++s) {
             const int row = s * kWave + lane + 1;
-            double MM = 0, GM = 0, MI = 0, II = 0, MD = 0, DD = 0, pm = 0, px = 0;
-            uint32_t x = 0, xbit = 0;
-            if (row <= R) {
-                const int p = row - 1;
-                x = b.rd_bases[ro + p];
-                const int q = b.rd_quals[ro + p], qi = b.rd_ins[ro + p], qd = b.rd_del[ro + p],
-                          qc = b.rd_gcp[ro + p];
-                const int mn = qi < qd ? qi : qd, mx = qi < qd ? qd : qi;
-                MM = tab.mm_prob[((mx * (mx + 1)) >> 1) + mn];
-                MI = tab.qual_err[qi];
-                MD = tab.qual_err[qd];
-                II = tab.qual_err[qc];
-                DD = II;
-                GM = 1.0 - II;
-                const double eq = tab.qual_err[q];
-                pm = 1.0 - eq;
-                px = b.tristate ? eq / 3.0 : eq;
-                if (x == 'N') px = pm;
-                xbit = read_base_bit(x);
-            }
+            const PdRow w = pd_row(b, tab, ro, R, row - 1);
+            const PdRow wn = pd_row(b, tab, ro, R, row);
             const uint32_t inside_bit = row == 1 ? kPdInsideRow1 : kPdInsideLater;
-            // Lane state: own E and branch at column c-1, the E of the row above at c-1 (diagonal),
-            // the column code.
-            double EM = 0, EI = 0, ED = 0, bM = 0, bI = 0, bD = 0;
-            double dM = 0, dI = 0, dD = 0;
-            uint32_t code = 0;
-            PdRecord nxt = rec[1];
-            if (lane == 0) {
-                const PdRecord c0 = rec[0];
-                dM = c0.em;
-                dI = c0.ei;
-                dD = c0.ed;
-            }
-            const bool last = s == n_stripes - 1;
-            const int steps = H + kWave - 1;
-            for (int t = 0; t < steps; ++t) {
-                const PdRecord cur = nxt;
-                nxt = rec[t + 2];
-                code = shr1_keep(cur.code, code);
-                const double uM = wave_shr1(cur.em, EM), uI = wave_shr1(cur.ei, EI), uD = wave_shr1(cur.ed, ED);
-                const bool match = ((code & 0xFFu) == x) || (code & xbit) != 0u;
-                const double prior = match ? pm : px;
-                const double M = prior * fma(dD, GM, fma(dI, GM, dM * MM));
-                const double I = fma(uI, II, uM * MI);
-                const double D = fma(ED, DD, EM * MD);
-                const bool inside = (code & inside_bit) != 0u;
-                bM = inside ? bM : EM;
-                bI = inside ? bI : EI;
-                bD = inside ? bD : ED;
-                const double f = (code & kPdDelEnd) ? 1.0 : 0.0;  // max(0, X) == X for the cells here
-                EM = fmax(bM * f, M);
-                EI = fmax(bI * f, I);
-                ED = fmax(bD * f, D);
-                dM = uM;
-                dI = uI;
-                dD = uD;
-                const int c = t - lane + 1;
-                if (last && row == R && c >= 1 && c <= H) acc += M + I;
-                if (lane == kWave - 1 && c >= 0 && c <= H) {
-                    rec[c].em = EM;
-                    rec[c].ei = EI;
-                    rec[c].ed = ED;
-                }
-            }
+            if (s == n_stripes - 1)
+                acc = pd_stripe<true>(rec, cds, lane, H, R, row, w, wn, inside_bit);
+            else
+                (void)pd_stripe<false>(rec, cds, lane, H, R, row, w, wn, inside_bit);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -176,7 +237,7 @@ __global__ __launch_bounds__(kWave) void pd_forward_kernel(PdBatch b, DeviceTabl
 int pd_record_columns(int max_hap_len) { return max_hap_len + kPdPadColumns; }
 
 size_t pd_lds_bytes(int max_hap_len) {
-    return static_cast<size_t>(pd_record_columns(max_hap_len)) * sizeof(PdRecord);
+    return static_cast<size_t>(kPdLowPad + pd_record_columns(max_hap_len)) * (sizeof(PdRecord) + sizeof(uint32_t));
 }
 
 hipError_t launch_pd_forward(const PdBatch &batch, const DeviceTables &tables, uint32_t n_pairs, int max_hap_len,

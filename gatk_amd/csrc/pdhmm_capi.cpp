@@ -50,6 +50,9 @@ void pd_column_codes(const uint8_t *bases, const uint8_t *pd, int H, uint32_t *c
     }
     const PdState end1 = pd_state_pass(pd, H, PdState::Normal, kPdInsideRow1, code);
     pd_state_pass(pd, H, end1, kPdInsideLater, code);
+    constexpr uint32_t inside = kPdInsideRow1 | kPdInsideLater;
+    for (int j = 0; j < H; ++j)
+        if ((code[j] & (inside | kPdDelEnd)) || (j + 1 < H && (code[j + 1] & inside))) code[j] |= kPdSpecial;
 }
 
 bool is_pd_base(uint8_t x) {

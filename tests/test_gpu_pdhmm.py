@@ -112,6 +112,26 @@ def test_edge_shapes():
     _check(reads, haps, "edge shapes")
 
 
+def test_dense_deletion_spans():
+    """Many overlapping deletion spans (0-24 per haplotype, 1-9 bases), a span opening at the first
+    base, DEL_END on the first base and spans ending on the last one, with reads over several
+    stripes: blocks with and without a branch column alternate often, so a branch value left
+    behind on a block without one would surface."""
+    rng = np.random.default_rng(16)
+    haps = [_pd_haplotype(rng, int(rng.integers(40, 400)), int(rng.integers(0, 5)), int(rng.integers(0, 25)),
+                          trailing_start=bool(k % 3 == 0)) for k in range(8)]
+    hp = bytearray(haps[0][1])
+    hp[0] |= DEL_START
+    haps[0] = (haps[0][0], bytes(hp))
+    hp = bytearray(haps[1][1])
+    hp[0] |= DEL_END
+    hp[-1] |= DEL_END
+    haps[1] = (haps[1][0], bytes(hp))
+    reads = [_read(rng, int(R), haps[k % len(haps)][0])
+             for k, R in enumerate([1, 64, 65, 200, 320] + list(rng.integers(1, 330, 11)))]
+    _check(reads, haps, "dense deletion spans")
+
+
 def test_long_haplotypes_and_reads():
     rng = np.random.default_rng(12)
     haps = [_pd_haplotype(rng, H, 8, 5) for H in (1000, 1500, 2000)]
