@@ -198,11 +198,16 @@ class DeviceBatch:
     def __init__(self, engine: "BatchEngine", batch: PackedBatch) -> None:
         self.engine = engine
         self.n_pairs = batch.n_pairs
-        self.cells = batch.cells()
+        self._batch = batch  # host copy, for `cells` on demand (RunStats carries the count too)
         self._h = ctypes.c_void_p()
         d = batch.desc()
         _raise_for(engine.lib.pairhmm_batch_upload(engine.ctx, ctypes.byref(d), ctypes.byref(self._h)), "batch_upload")
         engine._batches.add(self)  # a batch is freed before its engine's context (pairhmm_batch_free reads it)
+
+    @property
+    def cells(self) -> int:
+        """Sum over pairs of R * H (computed on the host when asked)."""
+        return self._batch.cells()
 
     def forward(self, mode: int = _native.RUN_PERSISTENT, out: Optional[np.ndarray] = None) -> RunStats:
         st = _native.RunStats()

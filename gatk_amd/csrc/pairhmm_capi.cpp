@@ -320,12 +320,28 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
     cells = 0;
     std::vector<uint32_t> cost(static_cast<size_t>(np));
     uint32_t max_cost = 0;
+    // LDS class of every haplotype, once (a batch has far fewer haplotypes than pairs), and the pairs
+    // and longest haplotype per class, counted in the pair loop.  top < 0: a haplotype too long for
+    // any class, reported after the pair checks.
+    const int top = lds_class_index(max_h);
+    std::vector<uint8_t> hap_cls;
+    std::vector<int64_t> n_in(static_cast<size_t>(std::max(top, 0)) + 1, 0);
+    std::vector<int> h_in(static_cast<size_t>(std::max(top, 0)) + 1, 1);
+    if (top >= 0) {
+        hap_cls.resize(static_cast<size_t>(d->n_haps));
+        for (int64_t h = 0; h < d->n_haps; ++h) hap_cls[h] = static_cast<uint8_t>(lds_class_index(d->hap_len[h]));
+    }
     for (int64_t p = 0; p < np; ++p) {
         const int32_t r = d->pair_read[p], h = d->pair_hap[p];
         if (r < 0 || r >= d->n_reads || h < 0 || h >= d->n_haps)
             return fail(PAIRHMM_ERR_BAD_ARG, "pair " + std::to_string(p) + " indexes outside the batch");
         const int R = d->read_len[r], H = d->hap_len[h];
         cells += static_cast<int64_t>(R) * H;
+        if (top >= 0) {
+            const int k = hap_cls[h];
+            n_in[k]++;
+            h_in[k] = std::max(h_in[k], H);
+        }
         // Aligned schedule: the pair's wave-steps, ceil(R/64) stripes of H + 64 steps.  Packed: the
         // haplotype length, so the pairs a wave takes one after another share stripe lengths.
         const uint32_t c = by_hap_len ? static_cast<uint32_t>(H)
@@ -334,7 +350,7 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         max_cost = std::max(max_cost, c);
     }
     if (np == 0) return PAIRHMM_OK;
-    if (lds_class_index(max_h) < 0)
+    if (top < 0)
         return fail(PAIRHMM_ERR_BAD_ARG, "haplotype of " + std::to_string(max_h) + " bases exceeds the LDS budget");
     if (queue_order == 1) {
         // Submission order, one slice; an FP64-only class sends every pair to FP64.
@@ -349,15 +365,6 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         for (int64_t p = 0; p < np; ++p) start[max_cost - cost[p] + 1]++;
         for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
         for (int64_t p = 0; p < np; ++p) by_cost[start[max_cost - cost[p]]++] = static_cast<uint32_t>(p);
-    }
-    const int top = lds_class_index(max_h);
-    std::vector<int64_t> n_in(static_cast<size_t>(top) + 1, 0);
-    std::vector<int> h_in(static_cast<size_t>(top) + 1, 1);
-    for (int64_t p = 0; p < np; ++p) {
-        const int H = d->hap_len[d->pair_hap[p]];
-        const int k = lds_class_index(H);
-        n_in[k]++;
-        h_in[k] = std::max(h_in[k], H);
     }
     // Classes with few pairs are merged into the next larger one: a launch per class costs a drain
     // of the persistent grid, worth it only for a class with real work.  The last FP32 class always
@@ -389,6 +396,8 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
         }
     }
     const int n_runs = static_cast<int>(run_class.size());
+    std::vector<uint8_t> hap_run(static_cast<size_t>(d->n_haps));
+    for (int64_t h = 0; h < d->n_haps; ++h) hap_run[h] = static_cast<uint8_t>(run_of[hap_cls[h]]);
     std::vector<int64_t> run_n(static_cast<size_t>(n_runs), 0), run_direct(static_cast<size_t>(n_runs), 0);
     std::vector<int> run_h(static_cast<size_t>(n_runs), 1);
     for (int k = 0; k <= top; ++k) {
@@ -406,7 +415,7 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
                 read_direct[r] = read_routes_direct(d->read_quals + d->read_off[r], d->read_len[r]);
         direct.resize(static_cast<size_t>(np));
         for (int64_t p = 0; p < np; ++p) {
-            const int r = run_of[lds_class_index(d->hap_len[d->pair_hap[p]])];
+            const int r = hap_run[d->pair_hap[p]];
             direct[p] = read_direct[d->pair_read[p]] || kColClasses[run_class[r]] > kMaxFp32Cols;
             if (direct[p]) run_direct[r]++;
         }
@@ -423,7 +432,7 @@ int32_t validate_and_order(const pairhmm_batch_desc *d, int queue_order, std::ve
     }
     for (int64_t i = 0; i < np; ++i) {
         const uint32_t p = by_cost[i];
-        const int r = run_of[lds_class_index(d->hap_len[d->pair_hap[p]])];
+        const int r = hap_run[d->pair_hap[p]];
         if (!direct.empty() && direct[p])
             order[pos_direct[r]++] = p;
         else
