@@ -63,11 +63,15 @@ int32_t pairhmm_initialize(const pairhmm_native_arguments *args, pairhmm_context
     static const ModelTables tables;  // built once per process (C++ magic static)
     if (e == hipSuccess) e = hipMalloc(&ctx->d_qual_err, sizeof(tables.qual_err));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_mm_prob, sizeof(double) * tables.mm_prob.size());
+    // Copies on the context's own stream, never the legacy null stream: a legacy-stream operation
+    // while another thread captures a graph on its context invalidates that capture.
     if (e == hipSuccess)
-        e = hipMemcpy(ctx->d_qual_err, tables.qual_err, sizeof(tables.qual_err), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(ctx->d_qual_err, tables.qual_err, sizeof(tables.qual_err), hipMemcpyHostToDevice,
+                           ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpy(ctx->d_mm_prob, tables.mm_prob.data(), sizeof(double) * tables.mm_prob.size(),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(ctx->d_mm_prob, tables.mm_prob.data(), sizeof(double) * tables.mm_prob.size(),
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         pairhmm_done(ctx);
         return fail(e == hipErrorOutOfMemory ? PAIRHMM_ERR_OOM : PAIRHMM_ERR_KERNEL,
@@ -557,9 +561,19 @@ extern "C" int32_t pairhmm_batch_upload(pairhmm_context *ctx, const pairhmm_batc
                   {o_pr, d->pair_read, 4 * (size_t)np},          {o_ph, d->pair_hap, 4 * (size_t)np},
                   {o_ord, order.data(), 4 * (size_t)np},         {o_res, rescue_init.data(), 4 * (size_t)np},
                   {o_cinit, counter_init.data(), 4 * (size_t)kCounterSlots}};
+    // On the context's stream, not the legacy null stream (see pairhmm_initialize); the sources are
+    // the caller's pageable arrays, borrowed for this call only, so the copies complete before return.
     for (const Cp &c : copies) {
         if (c.n == 0) continue;
-        hipError_t e = hipMemcpy(m + c.o, c.src, c.n, hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpyAsync(m + c.o, c.src, c.n, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(ctx->stream);
+            pairhmm_batch_free(bt);
+            return fail(PAIRHMM_ERR_KERNEL, std::string("batch upload: ") + hipGetErrorString(e));
+        }
+    }
+    {
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
             pairhmm_batch_free(bt);
             return fail(PAIRHMM_ERR_KERNEL, std::string("batch upload: ") + hipGetErrorString(e));
