@@ -29,10 +29,13 @@
 // and the row below takes A two steps later and B one step later by DPP wave_shr:1: two doubles
 // cross lanes per step instead of the three E values.  Lane 0 reads its A and B from an LDS record
 // per column that lane 63 of the previous stripe wrote (stripe 0 starts from row 0: M = I = 0,
-// D = 2^1020 / H, so A(1,c) = D GM_1 and B(1,c) = 0); lane 0 also reads the column code, which rides
-// down the wave with the values.  Records are 16 bytes (A, B) plus a 4-byte code array, so a wave
-// holds 20 bytes of LDS per column.  The row-R lane sums M + I over columns 1..H in column order, as
-// the reference does (:132-139), in the last stripe only.
+// D = 2^1020 / H, so A(1,c) = D GM_1 and B(1,c) = 0); every lane reads its own column's code from an
+// LDS array.  Records are 16 bytes (A, B) plus the 4-byte code, so a wave holds 20 bytes of LDS per
+// column.  Steps run in blocks of 4; a block in which no lane is on a special column (the host's
+// kPdSpecial: a deletion end, a column inside a span, or the column before a span) skips the branch
+// update and hands on E = X, which is exact because a row's branch values are only read on special
+// columns and the column before each span sets them from E.  The row-R lane sums M + I over
+// columns 1..H in column order, as the reference does (:132-139), in the last stripe only.
 #include "pairhmm_device.hpp"
 #include "pairhmm_kernels.hpp"
 
@@ -51,11 +54,6 @@ static_assert(sizeof(PdRecord) == 16, "PdRecord is one 16-byte LDS word");
 constexpr int kPdPadColumns = 68;  // records past H that lane 0 reads (c <= H + 66 with 4-step blocks)
 constexpr int kPdLowPad = 64;      // records below column 0, where lane 63's stores of columns < 0 land
 constexpr int kPdU = 4;            // wave-steps per unrolled block
-
-__device__ __forceinline__ uint32_t shr1_keep(uint32_t old, uint32_t src) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(src),
-                                                             kDppWaveShr1, 0xF, 0xF, false));
-}
 
 // Read-base class bit (placed like the column code's alternative mask) of isBasePDMatching
 // (:179-195): A/a, C/c, G/g, T/t, and 0x80 for any other base, which only an 'N' haplotype
