@@ -309,7 +309,9 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
     for (int s = 0; s < n_stripes; ++s) {
         const RowTrans<double> w = row_trans<double>(s * kWave + lane, R, rb, rq, ri, rd, rg, tab);
         const RowTrans<double> wn = row_trans<double>(s * kWave + lane + 1, R, rb, rq, ri, rd, rg, tab);
-        double m1 = 0, d1 = 0;
+        // dd = D / MD: D(c) = D(c-1) DD + M(c-1) MD becomes dd(c) = dd(c-1) DD + M(c-1), and
+        // S = I + D = I + MD dd, one FMA each (rows past R have MD = DD = 0, so S = I there).
+        double m1 = 0, dd1 = 0;
         double an1 = 0, an2 = 0, bn1 = 0;  // sums for the row below, one and two steps back
         // The previous stripe's (or the initial) stores complete before this stripe reads them.
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -330,8 +332,8 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 a = wave_shr1(a, an2);  // lane 0 keeps its plane values
                 i = wave_shr1(i, bn1);
                 const double m = a * prior;
-                const double d = fma(d1, w.DD, m1 * w.MD);
-                const double sx = i + d;
+                const double dd = fma(dd1, w.DD, m1);
+                const double sx = fma(dd, w.MD, i);
                 const double an = fma(sx, wn.GM, m * wn.MM);
                 const double bn = fma(i, wn.II, m * wn.MI);
                 wr[2 * u] = bn;
@@ -340,7 +342,7 @@ __device__ double forward_pair_f64(const WaveLds<double> &L, int lane, int H, in
                 an1 = an;
                 bn1 = bn;
                 m1 = m;
-                d1 = d;
+                dd1 = dd;
                 // The next block's halves, loaded unconditionally (the slice has slots past the last
                 // block): with a fixed count of loads in the loop, the wait for a half is a counted
                 // one instead of a full drain at the block start.
