@@ -95,7 +95,9 @@ __device__ __forceinline__ PdRow pd_row(const PdBatch &b, const DeviceTables &ta
 }
 
 // Lane state of a stripe: own E and branch values at column c-1, the row-below sums of the last
-// two steps, the last-row sum.
+// two steps, the last-row sum.  The D values (ED, bD) are carried divided by the row's MD:
+// D(c) = E_D(c-1) DD + E_M(c-1) MD becomes D/MD = (E_D/MD) DD + E_M, one FMA, and max() commutes with
+// the positive scale (rows past R have MD = DD = 0 and stay 0).
 struct PdLane {
     double EM = 0, EI = 0, ED = 0, bM = 0, bI = 0, bD = 0;
     double an1 = 0, an2 = 0, bn1 = 0, acc = 0;
@@ -112,7 +114,7 @@ __device__ __forceinline__ void pd_step(PdLane &st, const PdRecord &cur, uint32_
     const double I = wave_shr1(cur.b, st.bn1);
     const bool match = ((code & 0xFFu) == w.x) || (code & w.xbit) != 0u;
     const double M = (match ? w.pm : w.px) * a;
-    const double D = fma(st.ED, w.DD, st.EM * w.MD);
+    const double D = fma(st.ED, w.DD, st.EM);  // D / MD
     if constexpr (kBranch) {
         const bool inside = (code & inside_bit) != 0u;
         st.bM = inside ? st.bM : st.EM;
@@ -127,7 +129,7 @@ __device__ __forceinline__ void pd_step(PdLane &st, const PdRecord &cur, uint32_
         st.EI = I;
         st.ED = D;
     }
-    const double an = fma(st.EI + st.ED, wn.GM, st.EM * wn.MM);
+    const double an = fma(fma(st.ED, w.MD, st.EI), wn.GM, st.EM * wn.MM);
     const double bn = fma(st.EI, wn.II, st.EM * wn.MI);
     st.an2 = st.an1;
     st.an1 = an;
