@@ -315,7 +315,8 @@ typedef struct pairhmm_pd_haplotype_data_holder {
  * (VectorLoglessPairPDHMM.java:55-60) map to pairhmm_load / pairhmm_initialize / pairhmm_done; the
  * PDHMMNativeArguments (thread count, AVX level, OpenMP setting) have no device meaning.  A read
  * base outside ACGTacgtN meeting a SNP column whose base is neither it nor 'N' is
- * PAIRHMM_ERR_BAD_ARG, where LoglessPDPairHMM.isBasePDMatching throws (:195-196).  Synchronous. */
+ * PAIRHMM_ERR_BAD_ARG, where LoglessPDPairHMM.isBasePDMatching throws (:195-196), and so is a
+ * haplotype of more than 8,060 bases (one wave's LDS holds 20 bytes per column).  Synchronous. */
 int32_t pairhmm_pd_compute_likelihoods(pairhmm_context *ctx, const pairhmm_read_data_holder *reads,
                                        int32_t n_reads, const pairhmm_pd_haplotype_data_holder *haplotypes,
                                        int32_t n_haplotypes, double *likelihoods);
