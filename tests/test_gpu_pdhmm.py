@@ -139,6 +139,17 @@ def test_long_haplotypes_and_reads():
     _check(reads, haps, "long")
 
 
+def test_haplotype_length_limit():
+    """One wave's LDS holds 20 bytes per column plus pads: 8,060 bases fit (and match the oracle),
+    8,061 are rejected as the header states."""
+    rng = np.random.default_rng(17)
+    hap = _pd_haplotype(rng, 8060, 4, 3)
+    reads = [_read(rng, R, hap[0]) for R in (1, 70)]
+    _check(reads, [hap], "8,060-base haplotype")
+    with pytest.raises(ValueError):
+        _device(reads, [_pd_haplotype(rng, 8061, 0, 0)])
+
+
 def test_no_pd_bits_matches_the_fp64_drop_in():
     """Without PD bits the PD kernel is the plain LoglessPairHMM: compare with the engine's own FP64
     drop-in (use_double_precision) on the same inputs."""
