@@ -168,9 +168,10 @@ __device__ __forceinline__ T fwd_step(const Row<T> &w, T px, uint32_t hb, T a0, 
         i = is_first ? b0 : i;
     }
     const T m = a * prior;
-    T d = m1 * w.MD;
-    d = fma(d1, w.DD, d);
-    const T sx = i + d;
+    // D carried divided by MD: D(c) = D(c-1) DD + M(c-1) MD becomes d(c) = d(c-1) DD + M(c-1), and
+    // S = I + MD d, one FMA each (rows past R have MD = DD = 0, so d = 0 and S = I there).
+    const T d = fma(d1, w.DD, m1);
+    const T sx = fma(d, w.MD, i);
     m_out = m;
     i_out = i;
     d_out = d;
@@ -334,7 +335,7 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
 #pragma unroll
             for (int k = 0; k < kCk; ++k) nxt[k] = ckbase[(static_cast<size_t>(prev) * kCk + k) * kWave];
             T m1 = cur[0], i1 = cur[1], d1 = cur[2], m2 = cur[3], s2 = cur[4];
-            T s1 = i1 + d1;
+            T s1 = fma(d1, w.MD, i1);
             const T m_before = m1, d_before = d1;
             T av[kTile], bv[kTile];
             __builtin_memcpy(av, __builtin_assume_aligned(L.bs + kLdsPad + t0, 16), sizeof(av));
@@ -403,7 +404,8 @@ __device__ void grad_pair(const GradBatch &g, const GradLds<T, kCols> &L, T *__r
             const double ei = static_cast<double>(w.MI), ed = static_cast<double>(w.MD), ec = static_cast<double>(w.II);
             const double eb = 1.0 - static_cast<double>(w.pm);
             const double d_ei = (-gMM + gMI) * norm, d_ed = (-gMM + static_cast<double>(aMD)) * norm;
-            const double d_ec = (-gGM + gII + static_cast<double>(aDD)) * norm;
+            // aDD sums gD times D / MD (the forward's scaled D): d L / d DD = MD aDD.
+            const double d_ec = (-gGM + gII + static_cast<double>(aDD) * ed) * norm;
             const double d_eb = static_cast<double>(aE) * norm;
             atomicAdd(&g.g_qi[ro + p], d_ei * dq * ei);
             atomicAdd(&g.g_qd[ro + p], d_ed * dq * ed);
