@@ -93,13 +93,16 @@ READ_HOLDER_DTYPE = np.dtype([("read_bases", "<u8"), ("read_quals", "<u8"), ("in
 HAPLOTYPE_HOLDER_DTYPE = np.dtype([("haplotype_bases", "<u8"), ("length", "<i4"), ("_pad", "<i4")])
 
 
-def _concat_u8(items) -> np.ndarray:
-    """The items' bytes as one contiguous uint8 plane (uint8 ndarrays are joined directly)."""
+def _concat_u8(items, n_bytes: Optional[int] = None) -> np.ndarray:
+    """The items' bytes as one contiguous uint8 plane.  Byte-sized buffers (uint8 ndarrays, bytes)
+    are joined through the buffer protocol in one C loop; the result is kept only if it holds the
+    expected n_bytes (the items' lengths summed), so a wider dtype, a list or a string goes through
+    the per-item uint8 conversion instead."""
     try:
-        plane = np.concatenate(items)
-        if plane.dtype == np.uint8 and plane.ndim == 1:
-            return plane
-    except (ValueError, TypeError):
+        joined = b"".join(items)
+        if n_bytes is not None and len(joined) == n_bytes:
+            return np.frombuffer(joined, dtype=np.uint8)
+    except (TypeError, BufferError, ValueError):
         pass
     return np.concatenate([_as_u8(x) for x in items])
 
@@ -123,7 +126,7 @@ def marshal_planes(dtype: np.dtype, columns):
     total = int(lens.sum())
     # One concatenation for every column (column k's plane starts at k * total): a single call's
     # overhead instead of one per column.
-    plane = np.zeros(1, np.uint8) if n == 0 or total == 0 else _concat_u8(items)
+    plane = np.zeros(1, np.uint8) if n == 0 or total == 0 else _concat_u8(items, total * len(columns))
     base = plane.ctypes.data
     for k, name in enumerate(dtype.names[:len(columns)]):
         holders[name] = np.uint64(base + k * total) + offs
