@@ -478,10 +478,14 @@ def chunked_leg(args, chunk_regions=500):
     from gatk_amd.stream import _make_chunk
 
     n = args.chunked_regions
-    n_chunks = (n + chunk_regions - 1) // chunk_regions
+    n_gen = (n + chunk_regions - 1) // chunk_regions
     g0 = time.perf_counter()
-    chunks = [_make_chunk((min(chunk_regions, n - k * chunk_regions), 4_000_000 + k, 0.10)) for k in range(n_chunks)]
+    distinct = [_make_chunk((min(chunk_regions, n - k * chunk_regions), 4_000_000 + k, 0.10)) for k in range(n_gen)]
     gen = time.perf_counter() - g0
+    # --chunked-passes > 1 runs the generated chunks again (every pass a fresh upload and forward):
+    # a larger streamed workload without generating more regions.
+    chunks = distinct * max(1, args.chunked_passes)
+    n_chunks = len(chunks)
     cells = pairs = rescued = 0
     gpu_ms = 0.0
     engines = [BatchEngine(), BatchEngine()]
@@ -512,7 +516,8 @@ def chunked_leg(args, chunk_regions=500):
     same = bool(np.array_equal(first_out, again, equal_nan=True))
     for e in engines:
         e.close()
-    d = {"regions": n, "chunk_regions": chunk_regions, "pairs": pairs, "cells": cells, "rescued_pairs": rescued,
+    d = {"regions": n * max(1, args.chunked_passes), "distinct_regions": n, "passes": max(1, args.chunked_passes),
+         "chunk_regions": chunk_regions, "pairs": pairs, "cells": cells, "rescued_pairs": rescued,
          "gpu_gcups": round(cells / (gpu_ms * 1e-3) / 1e9, 1), "wall_gcups_with_transfers": round(cells / wall / 1e9, 1),
          "generation_s": round(gen, 1), "first_chunk_matches_serial_run": same,
          "workload": "streaming mix: 90% batched_uneven-shaped, 10% precision_stress regions; batch API per "
@@ -628,6 +633,9 @@ def main() -> int:
     ap.add_argument("--chunked-regions", type=int, default=10_000,
                     help="regions of the chunked streaming-workload leg (90%% batched_uneven, 10%% STR-rich; "
                          "0 skips it)")
+    ap.add_argument("--chunked-passes", type=int, default=1,
+                    help="times the chunked leg runs its generated chunks (each pass uploads and computes every "
+                         "chunk again; 20 passes of 10,000 regions stream 200,000)")
     ap.add_argument("--readprep-regions", type=int, default=2_000,
                     help="regions whose reads the input-side leg prepares (0 skips it)")
     ap.add_argument("--readprep-sample", type=int, default=300)
