@@ -633,7 +633,7 @@ def main() -> int:
     ap.add_argument("--chunked-regions", type=int, default=10_000,
                     help="regions of the chunked streaming-workload leg (90%% batched_uneven, 10%% STR-rich; "
                          "0 skips it)")
-    ap.add_argument("--chunked-passes", type=int, default=1,
+    ap.add_argument("--chunked-passes", type=int, default=20,
                     help="times the chunked leg runs its generated chunks (each pass uploads and computes every "
                          "chunk again; 20 passes of 10,000 regions stream 200,000)")
     ap.add_argument("--readprep-regions", type=int, default=2_000,
